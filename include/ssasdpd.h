@@ -1,0 +1,262 @@
+/*
+ * ssasdpd.h - C ABI of the MI355X (gfx950) sSSA-SDPD engine.
+ *
+ * The reference engine is reached through a generated C++ translation unit
+ * and a subprocess (spatialpy/solvers/solver.py:100-158, 443-507, 553-569;
+ * spatialpy/solvers/c_base/ssa_sdpd-c-simulation-engine/propensity_file_template.cpp:101-142).
+ * This library replaces that generated program: the model is lowered to plain
+ * arrays (the same arrays the generator bakes into the program text,
+ * solver.py:208-299, 312-331, 375-419) and handed over through the
+ * descriptor below.  All pointers are caller-owned host memory, C-contiguous;
+ * the engine copies what it needs to the GPU in ssasdpd_create.
+ *
+ * Conventions: every function returns 0 (SSASDPD_OK) or a status code; the
+ * message for the last failure on the calling thread is ssasdpd_last_error().
+ * An engine handle is used by one host thread at a time.
+ */
+#ifndef SSASDPD_H
+#define SSASDPD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSASDPD_ABI_VERSION 1
+
+/* Status codes.  The reference prints and calls exit(1) for each failure
+ * class (particle.cpp:88-126, 189-206; simulate_rdme.cpp:262-294, 322-349,
+ * 368-400); here each one is a return code, raised as SimulationError by
+ * the Python Solver (solver.py:595-597). */
+enum ssasdpd_status {
+    SSASDPD_OK = 0,
+    SSASDPD_ERR_BAD_DESC = 1,            /* descriptor fails validation       */
+    SSASDPD_ERR_NAN_STATE = 2,           /* particle.cpp:88-126               */
+    SSASDPD_ERR_NEGATIVE_STATE = 3,      /* simulate_rdme.cpp:288-294,395-400 */
+    SSASDPD_ERR_PROPENSITY_OVERFLOW = 4, /* simulate_rdme.cpp:277-280         */
+    SSASDPD_ERR_DIFFUSION_OVERFLOW = 5,  /* simulate_rdme.cpp:342-346,381-388 */
+    SSASDPD_ERR_NAN_DIJ = 6,             /* particle.cpp:189-206              */
+    SSASDPD_ERR_HIP = 7,                 /* HIP runtime / no device           */
+    SSASDPD_ERR_TIMEOUT = 8,             /* solver.py:583-586                 */
+    SSASDPD_ERR_REPLAY_EXHAUSTED = 9,    /* replay stream shorter than needed */
+    SSASDPD_ERR_CAPACITY = 10,           /* a fixed-size buffer is too small  */
+    SSASDPD_ERR_UNSUPPORTED = 11
+};
+
+/* Mass-action propensity kinds, evaluated left to right in FP64 exactly as
+ * the generated C expressions (spatialpy/core/reaction.py:260-313):
+ *   0: (k*vol)   1: (k*X)   2: (((k*X)*Y)/vol)   3: (((k*X)*(X-1))/vol)
+ * X, Y are the species counts read as signed int (the generated functions
+ * take `const int *x`, solver.py:345).  The deterministic (ODE) right-hand
+ * side of the same reaction drops `vol` and the -1:  0: k  1: k*X
+ * 2: ((k*X)*Y)  3: ((k*X)*X)  (reaction.py:297-311, solver.py:160-190). */
+typedef struct ssasdpd_reaction {
+    int32_t kind;
+    int32_t a;            /* first reactant species index (kinds 1-3) */
+    int32_t b;            /* second reactant species index (kind 2)   */
+    int32_t pad;
+    double k;             /* rate constant                            */
+    uint64_t type_mask;   /* restrict_to: bit t set => allowed in particle type t;
+                             0 => no restriction (solver.py:353-368)  */
+} ssasdpd_reaction;
+
+/* Boundary-condition rule (spatialpy/core/boundarycondition.py:121-167),
+ * applied in array order, twice per step (simulate.cpp:88, 171).  A bound
+ * that the model leaves unset is passed as -inf / +inf. */
+enum ssasdpd_bc_target { SSASDPD_BC_CHEM = 0, SSASDPD_BC_V = 1, SSASDPD_BC_NU = 2, SSASDPD_BC_RHO = 3 };
+typedef struct ssasdpd_bc {
+    double lo[3];
+    double hi[3];
+    int32_t type;         /* particle type, or -1 for any              */
+    int32_t target;       /* enum ssasdpd_bc_target                    */
+    int32_t species;      /* deterministic species index (target CHEM) */
+    int32_t pad;
+    double value[3];      /* value[0] for scalar targets                */
+} ssasdpd_bc;
+
+/* Engine behaviour flags. */
+enum ssasdpd_flags {
+    /* Select reaction/species with rand1*totrate instead of the reference's
+     * rand1*srrate / rand1*sdrate (simulate_rdme.cpp:255-261, 317-321). */
+    SSASDPD_FLAG_TEXTBOOK_SELECTION = 1u << 0,
+    /* Order neighbour-list ties by ANN kd-tree visit order on moving domains
+     * too (always on for static domains). */
+    SSASDPD_FLAG_ANN_ORDER_MOVING = 1u << 1,
+    /* Skip the sSSA part (diffusion + reactions). */
+    SSASDPD_FLAG_NO_RDME = 1u << 2,
+    /* Record the event trace (see ssasdpd_get_trace). */
+    SSASDPD_FLAG_TRACE = 1u << 3
+};
+
+typedef struct ssasdpd_desc {
+    int32_t abi_version;      /* SSASDPD_ABI_VERSION                           */
+    int32_t n;                /* particles == voxels                           */
+    int32_t dimension;        /* 1, 2 or 3 (solver.py:407-413)                 */
+    int32_t n_types;          /* types 1..n_types (type 0 excluded)            */
+    int32_t n_species;        /* stochastic species S (0: RDME off)            */
+    int32_t n_reactions;      /* stochastic reactions R                        */
+    int32_t n_chem_species;   /* deterministic species (0: PDE part off)       */
+    int32_t n_chem_reactions; /* deterministic reactions                       */
+    int32_t static_domain;    /* model.staticDomain                            */
+    int32_t n_bcs;
+    int32_t n_output_steps;
+    uint32_t flags;           /* enum ssasdpd_flags                            */
+    uint32_t nt;              /* number of steps                               */
+    uint32_t pad0;
+    double dt, h, rho0, c0, P0;
+    double bbox[6];           /* xlo xhi ylo yhi zlo zhi                       */
+    double gravity[3];
+
+    /* particles (propensity_file_template.cpp:73-83) */
+    const double *x;          /* [n][3]                                         */
+    const double *v;          /* [n][3] or NULL (zero)                          */
+    const int32_t *type;      /* [n], 1..n_types                                */
+    const double *nu;         /* [n]                                            */
+    const double *mass;       /* [n]                                            */
+    const double *rho;        /* [n]                                            */
+    const int32_t *fixed;     /* [n], solidTag                                  */
+    const uint32_t *u0;       /* [n][n_species] (solver.py:211-219)             */
+    const double *C0;         /* [n][n_chem_species] or NULL (= u0)             */
+
+    /* chemistry (solver.py:208-286; core/model.py:187-268) */
+    const double *diffusion_matrix;   /* [n_species][n_types] species-major     */
+    const ssasdpd_reaction *reactions;/* [n_reactions]                          */
+    const int32_t *jcN;       /* [R+1] stoichiometry CSC column pointers        */
+    const int32_t *irN;       /* [nnz] species rows                             */
+    const int32_t *prN;       /* [nnz] net change                               */
+    const int32_t *jcG;       /* [S+R+1] dependency-graph CSC column pointers   */
+    const int32_t *irG;       /* [nnzG] reaction rows                           */
+    const int32_t *N_dense;   /* [S][R] row-major (chem RHS, model.cpp:183-189) */
+
+    const ssasdpd_bc *bcs;    /* [n_bcs]                                        */
+    const uint32_t *output_steps; /* [n_output_steps] (core/timespan.py:72-85)  */
+} ssasdpd_desc;
+
+/* Output snapshot buffers, caller-allocated with ssasdpd_num_snapshots()
+ * entries.  Any pointer may be NULL to skip that field. */
+typedef struct ssasdpd_outputs {
+    int32_t n_snapshots;      /* capacity in snapshots                          */
+    int32_t pad;
+    uint32_t *xx;             /* [n_snap][n][S]                                 */
+    double *C;                /* [n_snap][n][Sc]                                */
+    double *x;                /* [n_snap][n][3]                                 */
+    double *v;                /* [n_snap][n][3]                                 */
+    double *rho;              /* [n_snap][n]                                    */
+    double *bvf_phi;          /* [n_snap][n]                                    */
+    double *nu;               /* [n_snap][n]                                    */
+    uint32_t *steps;          /* [n_snap] step index each snapshot was taken at */
+} ssasdpd_outputs;
+
+/* Random stream for a run.  kind 0: seeded (GPU counter-based generator,
+ * statistically exact, not the reference's bit stream).  kind 1: replay --
+ * the engine consumes words[] in exactly the reference's draw order
+ * (simulate_rdme.cpp:253, 353; NRMConstant_v5.cpp:57, 98): a uniform draw
+ * uses words[i] as double(words[i])/2^64, an exponential draw uses expo[i]
+ * = -log(1 - canonical(words[i])) as computed by the caller's libm.
+ * ssasdpd_mt19937_64_stream() fills both arrays from a seed. */
+typedef struct ssasdpd_rng {
+    int32_t kind;
+    int32_t pad;
+    uint64_t seed;
+    const uint64_t *words;
+    const double *expo;
+    uint64_t n_words;
+} ssasdpd_rng;
+
+typedef struct ssasdpd_stats {
+    uint64_t steps;
+    uint64_t particle_steps;
+    uint64_t reactions;       /* total_reactions (simulate_rdme.cpp:310)        */
+    uint64_t diffusions;      /* total_diffusion (simulate_rdme.cpp:445)        */
+    uint64_t rng_words_used;  /* replay words consumed                          */
+    double ms_neighbors, ms_sdpd, ms_rates, ms_ssa, ms_total;
+    int32_t bad_particle;     /* NaN particle id, or -1                         */
+    int32_t max_degree;       /* widest neighbour list seen                     */
+} ssasdpd_stats;
+
+typedef struct ssasdpd_engine ssasdpd_engine;
+
+/* ---- lifecycle (replaces main(), propensity_file_template.cpp:101-142) -- */
+int ssasdpd_create(const ssasdpd_desc *d, ssasdpd_engine **out);
+void ssasdpd_destroy(ssasdpd_engine *e);
+const char *ssasdpd_last_error(void);
+int ssasdpd_abi_version(void);
+
+/* Number of snapshots a full run writes, following the output schedule of
+ * run_simulation (simulate_threads.cpp:231-289). */
+int ssasdpd_num_snapshots(const ssasdpd_desc *d, int32_t *n_snap);
+
+/* Reset the state to the descriptor's initial state and set the random
+ * stream (main(): rng seeding + init_all_particles + initialize_rdme,
+ * propensity_file_template.cpp:127-138, simulate_rdme.cpp:40-48). */
+int ssasdpd_reset(ssasdpd_engine *e, const ssasdpd_rng *rng);
+
+/* Advance n_steps steps (the body of the loop at simulate_threads.cpp:232-281
+ * without the output branch). */
+int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps);
+
+/* Full trajectory: reset + the step loop with snapshots (run_simulation,
+ * simulate_threads.cpp:171-312). */
+int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *rng, ssasdpd_outputs *out);
+
+/* Many independent trajectories of the same model in one launch per step
+ * (the reference runs them one after another, solver.py:547-605).
+ * counts: [n_traj][n_snap][n][S]. */
+int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
+                      uint32_t *counts, int32_t n_snap);
+
+int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
+
+/* ---- state access (parity tests) ---------------------------------------- */
+typedef struct ssasdpd_state {
+    double *x, *v, *vt;       /* [n][3] each (any may be NULL)                  */
+    double *F, *Fbp;          /* [n][3]                                         */
+    double *rho, *Frho, *bvf_phi, *nu, *old_rho; /* [n]                          */
+    double *C, *Q;            /* [n][Sc]                                        */
+    uint32_t *xx;             /* [n][S]                                         */
+    double *srrate, *sdrate;  /* [n]                                            */
+    double *rrate;            /* [n][R]                                         */
+    double *Ddiag;            /* [n][S]                                         */
+    double *next_time;        /* [n] event queue firing times                   */
+} ssasdpd_state;
+int ssasdpd_get_state(ssasdpd_engine *e, ssasdpd_state *st);
+
+/* Current neighbour lists in list order (particle.cpp:150-210, 240-294):
+ * row_ptr [n+1]; idx/dist/dWdr/Dij [row_ptr[n]].  Pass idx == NULL to get
+ * only row_ptr (and hence the total). */
+int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx,
+                          double *dist, double *dWdr, double *Dij);
+
+/* Stand-alone fixed-radius search: tree points pts[n][3], queries q[m][3],
+ * dimension dim, radius h.  Same acceptance rule and list order as
+ * Particle::find_neighbors on a tree built from pts.  row_ptr [m+1];
+ * idx/r2 sized by the total in row_ptr[m] (call once with idx == NULL). */
+int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m,
+                      int32_t dim, double h, int64_t *row_ptr, int32_t *idx,
+                      double *r2);
+
+/* Event trace of the last run / advance when SSASDPD_FLAG_TRACE is set:
+ * per event (time, voxel, kind 0=reaction 1=diffusion, reaction or species,
+ * destination or -1). */
+typedef struct ssasdpd_event {
+    double t;
+    int32_t voxel;
+    int32_t kind;
+    int32_t which;
+    int32_t dest;
+} ssasdpd_event;
+int ssasdpd_set_trace_capacity(ssasdpd_engine *e, uint64_t capacity);
+int ssasdpd_get_trace(ssasdpd_engine *e, ssasdpd_event *ev, uint64_t cap, uint64_t *n_events);
+
+/* ---- host utilities ------------------------------------------------------ */
+/* MT19937-64 stream as std::mt19937_64(seed) produces it, plus the
+ * exponential variate -log(1-canonical(w)) of every word computed with the
+ * host libm (libstdc++ exponential_distribution, random.h:4761-4767). */
+int ssasdpd_mt19937_64_stream(uint64_t seed, uint64_t n, uint64_t *words, double *expo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSASDPD_H */

@@ -1,0 +1,1299 @@
+/*
+ * oracle.cpp -- FP64 CPU restatement of SpatialPy's SSA-SDPD engine.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker /
+ * the CPU baseline.  The product (spatialpy_amd, libssasdpd.so) never links
+ * or calls it.
+ *
+ * It restates, in this project's own code, the algorithm of
+ *   ENG = spatialpy/solvers/c_base/ssa_sdpd-c-simulation-engine
+ *   ANN = ENG/external/ANN (ANN 1.1.2, locally modified: no self match)
+ * with every function citing the reference lines it follows.  Semantics are
+ * those of the reference run with one worker thread (-t 1): particles are
+ * processed in index order inside each substep.  Option `gauss_seidel=0`
+ * switches the one order-dependent read (neighbour rho inside the boundary
+ * volume fraction, ENG/src/model.cpp:285-293) to snapshot (Jacobi) order,
+ * which is what the GPU engine implements.
+ *
+ * Fields the reference leaves uninitialised (Particle::F, Fbp, Frho, vt;
+ * ENG/src/particle.cpp:70-83) start at 0 here, as they do in the GPU engine
+ * and in the reference-linked harness (oracle/ref_harness.cpp).
+ *
+ * Parity pins (what checks each part; see tests/):
+ *   kd-tree + FR search  : reference-linked harness fixtures (tie order),
+ *                          brute force (set and r^2 bits), ANN test files.
+ *   add_to_neighbor_list : closed forms, D_ij symmetry, lattice sums.
+ *   SDPD substeps        : harness fixtures (FP64, same op order).
+ *   NSM + NRM queue      : harness event-sequence fixtures, MT19937-64 KAT,
+ *                          conservation, stoichiometric balance.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../include/ssasdpd.h"
+
+namespace oracle {
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string &m) { g_err = m; return code; }
+
+/* ------------------------------------------------------------------------
+ * MT19937-64 (Matsumoto & Nishimura 2004; C++ std::mt19937_64, used by the
+ * reference at propensity_file_template.cpp:99, 127).
+ * ---------------------------------------------------------------------- */
+struct MT64 {
+    uint64_t mt[312];
+    int idx;
+    explicit MT64(uint64_t seed = 5489ULL) { seed_with(seed); }
+    void seed_with(uint64_t seed) {
+        mt[0] = seed;
+        for (int i = 1; i < 312; i++)
+            mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+        idx = 312;
+    }
+    void twist() {
+        const uint64_t UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL;
+        const uint64_t A = 0xB5026F5AA96619E9ULL;
+        for (int i = 0; i < 312; i++) {
+            uint64_t y = (mt[i] & UM) | (mt[(i + 1) % 312] & LM);
+            uint64_t v = mt[(i + 156) % 312] ^ (y >> 1);
+            if (y & 1ULL) v ^= A;
+            mt[i] = v;
+        }
+        idx = 0;
+    }
+    uint64_t next() {
+        if (idx >= 312) twist();
+        uint64_t y = mt[idx++];
+        y ^= (y >> 29) & 0x5555555555555555ULL;
+        y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+        y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+        y ^= (y >> 43);
+        return y;
+    }
+};
+
+/* Uniform draw rng()*1.0/rng.max() (ENG/src/simulate_rdme.cpp:253, 353). */
+static inline double word_to_uniform(uint64_t w) {
+    return (double)w * 1.0 / (double)0xFFFFFFFFFFFFFFFFULL;
+}
+/* std::exponential_distribution<double>(1) of one mt19937_64 word:
+ * -log(1 - generate_canonical<double,53>) (libstdc++ random.h:4761-4767,
+ * random.tcc:3348-3380: one draw divided by 2^64, clamped below 1). */
+static inline double word_to_expo(uint64_t w) {
+    double sum = (double)w * 1.0;
+    double tmp = 18446744073709551616.0;
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = std::nextafter(1.0, 0.0);
+    return -std::log(1.0 - ret) / 1.0;
+}
+
+/* The random stream: either the reference generator itself (kind 0 here:
+ * mt19937_64(seed) + host libm) or a supplied replay stream (kind 1). */
+struct Stream {
+    int kind = 0;
+    MT64 mt;
+    const uint64_t *words = nullptr;
+    const double *expo = nullptr;
+    uint64_t n = 0, pos = 0;
+    bool exhausted = false;
+    double uniform() {
+        if (kind == 0) { pos++; return word_to_uniform(mt.next()); }
+        if (pos >= n) { exhausted = true; return 0.5; }
+        return word_to_uniform(words[pos++]);
+    }
+    double exponential() {
+        if (kind == 0) { pos++; return word_to_expo(mt.next()); }
+        if (pos >= n) { exhausted = true; return 1.0; }
+        return expo[pos++];
+    }
+};
+
+/* ------------------------------------------------------------------------
+ * kd-tree: restatement of ANNkd_tree with the sliding-midpoint rule and
+ * bucket size 1 (ANN/src/kd_tree.cpp:314-405, kd_split.cpp:146-213,
+ * kd_util.cpp:73-90, 154-185, 291-319) and the fixed-radius search
+ * (ANN/src/kd_fix_rad_search.cpp:58-183, pr_queue_k.h:100-116).
+ * ---------------------------------------------------------------------- */
+struct KdTree {
+    struct Node {
+        int leaf;          /* 1 leaf, 0 split */
+        int pt;            /* leaf point index, -1 for the empty leaf */
+        int cd;            /* cut dimension */
+        double cv;         /* cut value */
+        double lo_b, hi_b; /* cell bounds along cd (cd_bnds) */
+        int child[2];
+    };
+    int dim = 0, n = 0;
+    std::vector<double> pts;   /* n*dim */
+    std::vector<int> pidx;
+    std::vector<Node> nodes;
+    int root = -1;
+    double bb_lo[3], bb_hi[3];
+
+    double PA(int i, int d) const { return pts[(size_t)pidx[i] * dim + d]; }
+
+    /* kd_util.cpp:154-168 */
+    double spread(int *pi, int cnt, int d) const {
+        double mn = pts[(size_t)pi[0] * dim + d], mx = mn;
+        for (int i = 1; i < cnt; i++) {
+            double c = pts[(size_t)pi[i] * dim + d];
+            if (c < mn) mn = c;
+            else if (c > mx) mx = c;
+        }
+        return mx - mn;
+    }
+    /* kd_util.cpp:170-185 */
+    void minmax(int *pi, int cnt, int d, double &mn, double &mx) const {
+        mn = pts[(size_t)pi[0] * dim + d];
+        mx = mn;
+        for (int i = 1; i < cnt; i++) {
+            double c = pts[(size_t)pi[i] * dim + d];
+            if (c < mn) mn = c;
+            else if (c > mx) mx = c;
+        }
+    }
+    /* Two-pass partition about cv (kd_util.cpp:291-319). */
+    void plane_split(int *pi, int cnt, int d, double cv, int &br1, int &br2) const {
+        auto V = [&](int i) { return pts[(size_t)pi[i] * dim + d]; };
+        int l = 0, r = cnt - 1;
+        for (;;) {
+            while (l < cnt && V(l) < cv) l++;
+            while (r >= 0 && V(r) >= cv) r--;
+            if (l > r) break;
+            std::swap(pi[l], pi[r]);
+            l++; r--;
+        }
+        br1 = l;
+        r = cnt - 1;
+        for (;;) {
+            while (l < cnt && V(l) <= cv) l++;
+            while (r >= br1 && V(r) > cv) r--;
+            if (l > r) break;
+            std::swap(pi[l], pi[r]);
+            l++; r--;
+        }
+        br2 = l;
+    }
+    /* sl_midpt_split (kd_split.cpp:146-213), ERR = 0.001 (ANNx.h). */
+    void sl_midpt_split(int *pi, int cnt, const double *lo, const double *hi,
+                        int &cut_dim, double &cut_val, int &n_lo) const {
+        const double ERR = 0.001;
+        double max_length = hi[0] - lo[0];
+        for (int d = 1; d < dim; d++) {
+            double length = hi[d] - lo[d];
+            if (length > max_length) max_length = length;
+        }
+        double max_spread = -1;
+        cut_dim = 0;
+        for (int d = 0; d < dim; d++) {
+            if ((hi[d] - lo[d]) >= (1 - ERR) * max_length) {
+                double spr = spread(pi, cnt, d);
+                if (spr > max_spread) { max_spread = spr; cut_dim = d; }
+            }
+        }
+        double ideal = (lo[cut_dim] + hi[cut_dim]) / 2;
+        double mn, mx;
+        minmax(pi, cnt, cut_dim, mn, mx);
+        if (ideal < mn) cut_val = mn;
+        else if (ideal > mx) cut_val = mx;
+        else cut_val = ideal;
+        int br1, br2;
+        plane_split(pi, cnt, cut_dim, cut_val, br1, br2);
+        if (ideal < mn) n_lo = 1;
+        else if (ideal > mx) n_lo = cnt - 1;
+        else if (br1 > cnt / 2) n_lo = br1;
+        else if (br2 < cnt / 2) n_lo = br2;
+        else n_lo = cnt / 2;
+    }
+    /* rkd_tree (kd_tree.cpp:314-358) */
+    int rkd(int *pi, int cnt, double *lo, double *hi) {
+        if (cnt <= 1) {
+            Node nd{};
+            nd.leaf = 1;
+            nd.pt = cnt == 0 ? -1 : pi[0];
+            nodes.push_back(nd);
+            return (int)nodes.size() - 1;
+        }
+        int cd, n_lo;
+        double cv;
+        sl_midpt_split(pi, cnt, lo, hi, cd, cv, n_lo);
+        double lv = lo[cd], hv = hi[cd];
+        hi[cd] = cv;
+        int l = rkd(pi, n_lo, lo, hi);
+        hi[cd] = hv;
+        lo[cd] = cv;
+        int h = rkd(pi + n_lo, cnt - n_lo, lo, hi);
+        lo[cd] = lv;
+        Node nd{};
+        nd.leaf = 0; nd.pt = -1; nd.cd = cd; nd.cv = cv; nd.lo_b = lv; nd.hi_b = hv;
+        nd.child[0] = l; nd.child[1] = h;
+        nodes.push_back(nd);
+        return (int)nodes.size() - 1;
+    }
+    /* ANNkd_tree constructor (kd_tree.cpp:368-405) with annEnclRect
+     * (kd_util.cpp:73-90). */
+    void build(const double *p, int npts, int d) {
+        dim = d; n = npts;
+        pts.assign(p, p + (size_t)npts * d);
+        pidx.resize(npts);
+        for (int i = 0; i < npts; i++) pidx[i] = i;
+        nodes.clear();
+        root = -1;
+        if (npts == 0) return;
+        for (int k = 0; k < d; k++) {
+            double lo = PA(0, k), hi = PA(0, k);
+            for (int i = 0; i < npts; i++) {
+                if (PA(i, k) < lo) lo = PA(i, k);
+                else if (PA(i, k) > hi) hi = PA(i, k);
+            }
+            bb_lo[k] = lo; bb_hi[k] = hi;
+        }
+        double lo[3], hi[3];
+        for (int k = 0; k < d; k++) { lo[k] = bb_lo[k]; hi[k] = bb_hi[k]; }
+        nodes.reserve(2 * (size_t)npts);
+        root = rkd(pidx.data(), npts, lo, hi);
+    }
+
+    /* search state (kd_fix_rad_search.cpp:45-52) */
+    struct Hit { double d; int i; };
+    mutable const double *Q = nullptr;
+    mutable double sqRad = 0;
+    mutable std::vector<Hit> *out = nullptr;
+
+    /* ANNkd_leaf::ann_FR_search (kd_fix_rad_search.cpp:148-183) with the
+     * stable insertion of ANNmin_k::insert (pr_queue_k.h:100-116). */
+    void leaf_search(const Node &nd) const {
+        if (nd.pt < 0) return;
+        const double *pp = &pts[(size_t)nd.pt * dim];
+        double dist = 0;
+        int d;
+        for (d = 0; d < dim; d++) {
+            double t = Q[d] - pp[d];
+            if ((dist = dist + t * t) > sqRad) break;
+        }
+        if (d >= dim && dist != 0) {
+            std::vector<Hit> &v = *out;
+            v.push_back({dist, nd.pt});
+            size_t i = v.size() - 1;
+            while (i > 0 && v[i - 1].d > dist) { v[i] = v[i - 1]; i--; }
+            v[i] = {dist, nd.pt};
+        }
+    }
+    /* ANNkd_split::ann_FR_search (kd_fix_rad_search.cpp:100-140) */
+    void search(int ni, double box_dist) const {
+        const Node &nd = nodes[ni];
+        if (nd.leaf) { leaf_search(nd); return; }
+        double cut_diff = Q[nd.cd] - nd.cv;
+        if (cut_diff < 0) {
+            search(nd.child[0], box_dist);
+            double box_diff = nd.lo_b - Q[nd.cd];
+            if (box_diff < 0) box_diff = 0;
+            box_dist = box_dist + (cut_diff * cut_diff - box_diff * box_diff);
+            if (box_dist * 1.0 <= sqRad) search(nd.child[1], box_dist);
+        } else {
+            search(nd.child[1], box_dist);
+            double box_diff = Q[nd.cd] - nd.hi_b;
+            if (box_diff < 0) box_diff = 0;
+            box_dist = box_dist + (cut_diff * cut_diff - box_diff * box_diff);
+            if (box_dist * 1.0 <= sqRad) search(nd.child[0], box_dist);
+        }
+    }
+    /* annkFRSearch (kd_fix_rad_search.cpp:58-89) with annBoxDistance
+     * (kd_util.cpp:124-146) for the root. */
+    void fr_search(const double *q, double sq, std::vector<Hit> &res) const {
+        res.clear();
+        if (root < 0) return;
+        Q = q; sqRad = sq; out = &res;
+        double dist = 0.0;
+        for (int d = 0; d < dim; d++) {
+            if (q[d] < bb_lo[d]) { double t = bb_lo[d] - q[d]; dist = dist + t * t; }
+            else if (q[d] > bb_hi[d]) { double t = q[d] - bb_hi[d]; dist = dist + t * t; }
+        }
+        search(root, dist);
+    }
+};
+
+/* ------------------------------------------------------------------------
+ * Particle system.
+ * ---------------------------------------------------------------------- */
+struct Nbr { int j; double dist, dWdr, Dij; };
+
+struct Sim {
+    /* configuration */
+    int n = 0, dim = 3, T = 1, S = 0, R = 0, Sc = 0, Rc = 0;
+    int static_domain = 1;
+    uint32_t nt = 0, flags = 0;
+    double dt = 0, h = 0, rho0 = 1, c0 = 10, P0 = 100;
+    double grav[3] = {0, 0, 0};
+    int gauss_seidel = 1;
+    std::vector<ssasdpd_reaction> rxn;
+    std::vector<int> jcN, irN, prN, jcG, irG, Ndense;
+    std::vector<double> Dmat;
+    std::vector<ssasdpd_bc> bcs;
+    std::vector<uint32_t> out_steps;
+
+    /* initial state */
+    std::vector<double> x0, v0, rho_init, C_init;
+    std::vector<uint32_t> u0;
+
+    /* particle state (ENG/include/particle.hpp:162-201) */
+    std::vector<int> type, solid;
+    std::vector<double> x, v, vt, F, Fbp, Frho, rho, old_rho, mass, nu, bvfphi, normal, C, Q;
+    std::vector<std::vector<Nbr>> nbrs;
+    /* RDME voxel state */
+    std::vector<uint32_t> xx;
+    std::vector<double> rrate, srrate, sdrate, Ddiag;
+    int initialized = 0;
+    uint64_t total_reactions = 0, total_diffusion = 0;
+    uint32_t current_step = 0;
+
+    KdTree tree;
+    bool tree_built = false;
+    Stream rng;
+
+    /* trace */
+    std::vector<ssasdpd_event> trace;
+    uint64_t trace_cap = 0;
+    uint64_t trace_n = 0;
+
+    int err = 0;
+    int bad_particle = -1;
+
+    /* NRM queue (ENG/src/NRMConstant_v5.cpp) */
+    struct Queue {
+        std::vector<double> next;                /* nextFiringTime */
+        std::vector<std::pair<int, int>> pos;    /* binIndexAndPositionInBin */
+        std::vector<std::vector<std::pair<double, int>>> table;
+        size_t minBin = 0, nbins = 0;
+        double L = 0, U = 0, endTime = 0;
+        size_t rxnCount = 0, active = 0;
+    } q;
+
+    /* -------------------------------------------------------------- */
+    double alpha() const {
+        /* ENG/src/particle.cpp:168-175 (1-D: 5 / 4 * h is integer 1 * h) */
+        if (dim == 3) return 105 / (16 * M_PI * h * h * h);
+        if (dim == 2) return 5 / (M_PI * h * h);
+        return 5 / 4 * h;
+    }
+
+    /* Particle::add_to_neighbor_list (ENG/src/particle.cpp:150-210) */
+    bool add_nbr(int i, int j, double r2) {
+        double r = std::sqrt(r2);
+        if (r > h) return true;
+        double R = r / h;
+        double a = alpha();
+        double dWdr = a * (-12 * r / (h * h)) * ((1 - R) * (1 - R));
+        double ih = 1.0 / h;
+        double ihsq = ih * ih;
+        double dhr = h - r;
+        double wfd = -25.066903536973515383e0 * dhr * dhr * ihsq * ihsq * ihsq * ih;
+        double Dij = -2.0 * (mass[i] * mass[j]) / (mass[i] + mass[j]) * (rho[i] + rho[j]) /
+                     (rho[i] * rho[j]) * r2 * wfd / (r2 + 0.01 * h * h);
+        if (std::isnan(Dij)) { bad_particle = i; err = SSASDPD_ERR_NAN_DIJ; return false; }
+        nbrs[i].push_back({j, r, dWdr, Dij});
+        return true;
+    }
+
+    /* buildKDTree (ENG/src/simulate_threads.cpp:80-108) */
+    void build_tree() {
+        if (tree_built && static_domain) return;
+        std::vector<double> p((size_t)n * dim);
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < dim; d++) p[(size_t)i * dim + d] = x[3 * (size_t)i + d];
+        tree.build(p.data(), n, dim);
+        tree_built = true;
+    }
+
+    /* Particle::find_neighbors (ENG/src/particle.cpp:240-294); the count
+     * pass and the fetch pass of annkFRSearch visit the same points, so one
+     * traversal gives the same list. */
+    bool find_neighbors(int i) {
+        double qp[3];
+        for (int d = 0; d < dim; d++) qp[d] = x[3 * (size_t)i + d];
+        std::vector<KdTree::Hit> hits;
+        tree.fr_search(qp, h * h, hits);
+        nbrs[i].clear();
+        for (auto &hh : hits)
+            if (!add_nbr(i, hh.i, hh.d)) return false;
+        return true;
+    }
+
+    void apply_bcs(int i) {
+        /* generated applyBoundaryConditions (propensity_file_template.cpp:92-94,
+         * spatialpy/core/boundarycondition.py:121-167) */
+        for (auto &b : bcs) {
+            const double *p = &x[3 * (size_t)i];
+            if (!(p[0] >= b.lo[0] && p[0] <= b.hi[0] && p[1] >= b.lo[1] && p[1] <= b.hi[1] &&
+                  p[2] >= b.lo[2] && p[2] <= b.hi[2]))
+                continue;
+            if (b.type >= 0 && type[i] != b.type) continue;
+            switch (b.target) {
+            case SSASDPD_BC_CHEM: if (b.species < Sc) C[(size_t)i * Sc + b.species] = b.value[0]; break;
+            case SSASDPD_BC_V: for (int k = 0; k < 3; k++) v[3 * (size_t)i + k] = b.value[k]; break;
+            case SSASDPD_BC_NU: nu[i] = b.value[0]; break;
+            case SSASDPD_BC_RHO: rho[i] = b.value[0]; break;
+            }
+        }
+    }
+
+    bool nan_check(int i) {
+        /* Particle::check_particle_nan (ENG/src/particle.cpp:88-134) */
+        for (int k = 0; k < 3; k++) {
+            double a = x[3 * (size_t)i + k], b = v[3 * (size_t)i + k];
+            if (std::isnan(a) || !std::isfinite(a) || std::isnan(b) || !std::isfinite(b)) goto bad;
+        }
+        if (std::isnan(rho[i]) || !std::isfinite(rho[i])) goto bad;
+        return true;
+    bad:
+        bad_particle = i;
+        err = SSASDPD_ERR_NAN_STATE;
+        return false;
+    }
+
+    /* take_step1 (ENG/src/simulate.cpp:56-109) */
+    void step1(int i, uint32_t step) {
+        if (solid[i] == 0 && static_domain == 0) {
+            for (int k = 0; k < 3; k++) {
+                size_t o = 3 * (size_t)i + k;
+                v[o] = v[o] + 0.5 * dt * F[o];
+                vt[o] = v[o] + 0.5 * dt * Fbp[o];
+                x[o] = x[o] + dt * vt[o];
+            }
+            rho[i] = rho[i] + 0.5 * dt * Frho[i];
+        }
+        if (step > 0)
+            for (int s = 0; s < Sc; s++) C[(size_t)i * Sc + s] += Q[(size_t)i * Sc + s] * dt * 0.5;
+        apply_bcs(i);
+        for (int k = 0; k < 3; k++) { F[3 * (size_t)i + k] = grav[k]; Fbp[3 * (size_t)i + k] = 0.0; }
+        Frho[i] = 0.0;
+        for (int s = 0; s < Sc; s++) Q[(size_t)i * Sc + s] = 0.0;
+        old_rho[i] = rho[i];
+    }
+
+    double chem_rhs(int i, int r) const {
+        /* generated det<name> functions (solver.py:160-190) */
+        const ssasdpd_reaction &re = rxn[r];
+        if (re.type_mask && !((re.type_mask >> type[i]) & 1ULL)) return 0.0;
+        const double *c = &C[(size_t)i * Sc];
+        switch (re.kind) {
+        case 0: return re.k;
+        case 1: return re.k * c[re.a];
+        case 2: return (re.k * c[re.a]) * c[re.b];
+        default: return (re.k * c[re.a]) * c[re.a];
+        }
+    }
+
+    /* pairwiseForce (ENG/src/model.cpp:39-191) */
+    void pairwise_force(int i) {
+        double dx[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
+        const double Pi = P0 * (rho[i] / rho0 - 1.0);
+        const double *xi = &x[3 * (size_t)i], *vi = &v[3 * (size_t)i], *vti = &vt[3 * (size_t)i];
+        for (const Nbr &nb : nbrs[i]) {
+            int j = nb.j;
+            double r = nb.dist, R = r / h;
+            if (R > 1.0) continue;
+            if (r == 0.0) continue;
+            double dWdr = nb.dWdr;
+            const double *xj = &x[3 * (size_t)j], *vj = &v[3 * (size_t)j], *vtj = &vt[3 * (size_t)j];
+            double dv_dx = 0.0;
+            for (int k = 0; k < dim; k++) {
+                dx[k] = xi[k] - xj[k];
+                dv[k] = vi[k] - vj[k];
+                dv_dx += dv[k] * dx[k];
+            }
+            double Pj = P0 * (rho[j] / rho0 - 1.0);
+            double pg = Pi / (rho[i] * rho[i]) + Pj / (rho[j] * rho[j]);
+            if (pg < 0) pg = -Pi / (rho[i] * rho[i]) + Pj / (rho[j] * rho[j]);
+            double fp = -1.0 * mass[j] * pg * dWdr / (r + 0.001 * h);
+            double fv = mass[j] * (2.0 * (nu[i] * nu[j]) / (nu[i] + nu[j])) * (1 / (r + 0.001 * h)) * dWdr /
+                        ((rho[i] * rho[j]));
+            double fbp = -10.0 * P0 * (1.0 / mass[i]) *
+                         (std::pow(mass[i] / rho[i], 2) + std::pow(mass[j] / rho[j], 2)) * dWdr / (r + 0.001 * h);
+            double T[3][3], ft[3];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    T[a][b] = 0.5 * ((rho[i] * vi[a] * (vti[b] - vi[b])) + (rho[j] * vj[a] * (vtj[b] - vj[b])));
+            for (int a = 0; a < 3; a++)
+                ft[a] = (1.0 / mass[i]) * (std::pow(mass[i] / rho[i], 2) + std::pow(mass[j] / rho[j], 2)) *
+                        (T[a][0] * dx[0] + T[a][1] * dx[1] + T[a][2] * dx[2]) * dWdr / (r + 0.001 * h);
+            for (int k = 0; k < dim; k++) {
+                F[3 * (size_t)i + k] += fp * dx[k] + fv * dv[k] + ft[k];
+                Fbp[3 * (size_t)i + k] += fbp * dx[k];
+            }
+            Frho[i] = Frho[i] + rho[i] * (mass[j] / rho[j]) * dv_dx * (1 / (r + 0.001 * h)) * dWdr -
+                      0.0 * h * rho[i] * c0 * mass[j] * 2.0 * (rho[j] / rho[i] - 1.0) * (r * r / (r * r + 0.01 * h * h)) *
+                          (1.0 / (r + 0.001 * h)) * dWdr / rho[j] -
+                      (mass[j] / rho[j]) *
+                          (rho[i] * ((vi[0] - vti[0]) * dx[0] + (vi[1] - vti[1]) * dx[1] + (vi[2] - vti[2]) * dx[2]) +
+                           rho[j] * ((vj[0] - vtj[0]) * dx[0] + (vj[1] - vtj[1]) * dx[1] + (vj[2] - vtj[2]) * dx[2])) *
+                          (1.0 / (r + 0.001 * h)) * dWdr;
+            double wfd = (1.0 / (r + 0.001 * h)) * dWdr;
+            double dQc_base = 2.0 * ((mass[i] * mass[j]) / (mass[i] + mass[j])) * ((rho[i] + rho[j]) / (rho[i] * rho[j])) *
+                              (r * r) * wfd / ((r * r) + 0.01 * h * h);
+            for (int s = 0; s < Sc; s++) {
+                size_t k = (size_t)Sc * (type[i] - 1) + s;   /* type-major index, model.cpp:163 */
+                double dq = (k < Dmat.size() ? Dmat[k] : 0.0) * (C[(size_t)i * Sc + s] - C[(size_t)j * Sc + s]) * dQc_base;
+                Q[(size_t)i * Sc + s] += dq;
+            }
+        }
+        double vol = mass[i] / rho[i];
+        (void)vol;
+        for (int r = 0; r < Rc; r++) {
+            double flux = chem_rhs(i, r);
+            for (int s = 0; s < Sc; s++) {
+                size_t k = (size_t)Rc * r + s;              /* model.cpp:186 */
+                int nk = k < Ndense.size() ? Ndense[k] : 0; /* past the end: 0 */
+                Q[(size_t)i * Sc + s] += nk * flux;
+            }
+        }
+    }
+
+    /* filterDensity (ENG/src/model.cpp:194-233) */
+    void filter_density(int i) {
+        double a = alpha(), num = 0.0, den = 0.0;
+        for (const Nbr &nb : nbrs[i]) {
+            double r = nb.dist, R = r / h;
+            if (R > 1.0) continue;
+            if (r == 0.0) continue;
+            double W = a * ((1 + 3 * R) * std::pow(1 - R, 3));
+            num += old_rho[nb.j] * W;
+            den += W;
+        }
+        rho[i] = num / den;
+    }
+
+    /* computeBoundaryVolumeFraction + applyBoundaryVolumeFraction
+     * (ENG/src/model.cpp:236-329).  rho_src is the array neighbour rho is
+     * read from (live array in Gauss-Seidel order, snapshot in Jacobi). */
+    void bvf_apply(int i, const std::vector<double> &rho_src) {
+        double a = alpha(), vos = 0.0, vtot = 0.0, nw[3] = {0, 0, 0}, dx[3] = {0, 0, 0};
+        for (int k = 0; k < dim; k++) { normal[3 * (size_t)i + k] = 0.0; vt[3 * (size_t)i + k] = 0.0; }
+        bvfphi[i] = 0.0;
+        for (const Nbr &nb : nbrs[i]) {
+            int j = nb.j;
+            double r = nb.dist, R = r / h;
+            if (R > 1.0) continue;
+            if (r == 0.0) continue;
+            double W = a * ((1 + 3 * R) * std::pow(1 - R, 3));
+            double dWdr = nb.dWdr;
+            for (int k = 0; k < dim; k++) dx[k] = x[3 * (size_t)i + k] - x[3 * (size_t)j + k];
+            double mv = std::pow(mass[j] / rho_src[j], 2);
+            if (solid[j]) vos += mv * W;
+            vtot += mv * W;
+            if (solid[j])
+                for (int k = 0; k < 3; k++) nw[k] += std::pow(mass[j] / rho_src[j], 2) * dx[k] * dWdr / (r + 0.001 * h);
+        }
+        for (int k = 0; k < 3; k++) nw[k] = nw[k] / vtot;
+        double nn = std::sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+        for (int k = 0; k < 3; k++) normal[3 * (size_t)i + k] = -nw[k] / nn;
+        bvfphi[i] = solid[i] ? 0.0 : std::fabs(vos / vtot);
+        /* applyBoundaryVolumeFraction */
+        const double *nm = &normal[3 * (size_t)i];
+        double *vi = &v[3 * (size_t)i];
+        double vdn = vi[0] * nm[0] + vi[1] * nm[1] + vi[2] * nm[2];
+        if (bvfphi[i] >= 0.5)
+            for (int k = 0; k < dim; k++) vi[k] = -vi[k] + 2.0 * std::fmax(0.0, vdn) * nm[k];
+    }
+
+    /* take_step2 (ENG/src/simulate.cpp:132-173), split in two phases so the
+     * BVF read of neighbour rho can be ordered either way. */
+    void step2a(int i, uint32_t step) {
+        if (solid[i] == 0 && static_domain == 0) {
+            for (int k = 0; k < 3; k++) v[3 * (size_t)i + k] = v[3 * (size_t)i + k] + 0.5 * dt * F[3 * (size_t)i + k];
+            if (step % 20 == 0) filter_density(i);
+            rho[i] = rho[i] + 0.5 * dt * Frho[i];
+        } else if (solid[i] == 1 && static_domain == 0) {
+            if (step % 20 == 0) filter_density(i);
+        }
+    }
+    void step2b(int i, const std::vector<double> &rho_src) {
+        if (solid[i] == 0 && static_domain == 0) bvf_apply(i, rho_src);
+        for (int s = 0; s < Sc; s++) C[(size_t)i * Sc + s] += Q[(size_t)i * Sc + s] * dt * 0.5;
+        apply_bcs(i);
+    }
+
+    /* ---------------- RDME ---------------- */
+    double propensity(int r, const uint32_t *xv, double vol, int tp) const {
+        /* generated propensity functions (solver.py:344-373,
+         * spatialpy/core/reaction.py:260-313) */
+        const ssasdpd_reaction &re = rxn[r];
+        if (re.type_mask && !((re.type_mask >> tp) & 1ULL)) return 0.0;
+        const int *xs = (const int *)xv;
+        switch (re.kind) {
+        case 0: return re.k * vol;
+        case 1: return re.k * xs[re.a];
+        case 2: return ((re.k * xs[re.a]) * xs[re.b]) / vol;
+        default: return ((re.k * xs[re.a]) * (xs[re.a] - 1)) / vol;
+        }
+    }
+    /* nsm_core__initialize_rxn_propensities (simulate_rdme.cpp:113-128) */
+    void init_rxn() {
+        for (int i = 0; i < n; i++) {
+            srrate[i] = 0.0;
+            for (int j = 0; j < R; j++) {
+                double vol = mass[i] / rho[i];
+                rrate[(size_t)i * R + j] = propensity(j, &xx[(size_t)i * S], vol, type[i]);
+                srrate[i] += rrate[(size_t)i * R + j];
+            }
+        }
+    }
+    /* nsm_core__initialize_diff_propensities (simulate_rdme.cpp:131-152) */
+    bool init_diff() {
+        for (int i = 0; i < n; i++) {
+            if (nbrs[i].empty())
+                if (!find_neighbors(i)) return false;
+            sdrate[i] = 0.0;
+            for (int s = 0; s < S; s++) {
+                double dd = 0.0;
+                for (const Nbr &nb : nbrs[i]) {
+                    double dc = Dmat[(size_t)s * T + (type[nb.j] - 1)];
+                    dd += dc * nb.Dij;
+                }
+                Ddiag[(size_t)i * S + s] = dd;
+                sdrate[i] += dd * xx[(size_t)i * S + s];
+            }
+        }
+        return true;
+    }
+
+    /* NRMConstant_v5::computeBinIndex (NRMConstant_v5.cpp:183-191) */
+    int bin_index(double t) const {
+        if (t > q.U) return -1;
+        int b = (int)((t - q.L) / (q.U - q.L) * q.nbins);
+        int rv = std::max(b, (int)q.minBin);
+        return std::min<int>(rv, (int)q.nbins - 1);
+    }
+    /* NRMConstant_v5::setBinNumberAndBounds (NRMConstant_v5.cpp:245-281) */
+    bool set_bins(double newL, double psum, int active) {
+        if (active == 0) return false;
+        q.L = newL;
+        if (q.L > q.endTime) return false;
+        double w = 16.0 / psum;
+        size_t maxb = (size_t)20.0 * std::sqrt(active);
+        size_t endb = std::numeric_limits<size_t>::max();
+        if (q.endTime < std::numeric_limits<double>::max()) {
+            endb = (size_t)std::ceil((q.endTime - q.L) / w);
+            q.nbins = std::min(endb, maxb);
+        } else {
+            q.nbins = maxb;
+        }
+        q.U = q.L + w * (double)q.nbins;
+        if (q.endTime < std::numeric_limits<double>::max()) {
+            if (q.U < q.endTime && ((double)(endb - maxb) / ((double)q.nbins)) < .2) {
+                q.nbins = endb;
+                q.U = q.L + w * (double)q.nbins;
+            }
+        }
+        return true;
+    }
+    void insert_all() {
+        q.table.assign(q.nbins, {});
+        for (int i = 0; i < n; i++) {
+            int b = bin_index(q.next[i]);
+            if (b >= 0) {
+                q.table[b].push_back({q.next[i], i});
+                q.pos[i] = {b, (int)q.table[b].size() - 1};
+            } else {
+                q.pos[i] = {-1, -1};
+            }
+        }
+    }
+    /* NRMConstant_v5::build (NRMConstant_v5.cpp:27-85) */
+    bool q_build(const std::vector<double> &prop, double psum, size_t active, double t0, double tend) {
+        q.active = active;
+        q.endTime = tend;
+        q.next.assign(n, 0.0);
+        q.pos.assign(n, {-1, -1});
+        if (psum == 0.0) return false;
+        if (!set_bins(t0, psum, (int)active)) return false;
+        q.minBin = 0;
+        size_t cnt = 0;
+        for (int i = 0; i < n; i++) {
+            double ft;
+            if (prop[i] == 0.0) ft = std::numeric_limits<double>::infinity();
+            else { ft = rng.exponential() / prop[i] + t0; ++cnt; }
+            q.next[i] = ft;
+        }
+        insert_all();
+        q.rxnCount = 0;
+        return cnt == q.active;
+    }
+    /* NRMConstant_v5::update (NRMConstant_v5.cpp:89-132) */
+    void q_update(int idx, double prop, double tnow) {
+        double ft;
+        const double inf = std::numeric_limits<double>::infinity();
+        if (prop <= 0.0) {
+            ft = inf;
+            if (q.next[idx] != inf) q.active--;
+        } else {
+            ft = rng.exponential() / prop + tnow;
+            if (q.next[idx] == inf) q.active++;
+        }
+        q.next[idx] = ft;
+        int nb = bin_index(ft);
+        int ob = q.pos[idx].first, op = q.pos[idx].second;
+        if (nb != ob) {
+            if (ob >= 0) {
+                auto &bin = q.table[ob];
+                if (bin.size() > 1) {
+                    bin[op] = bin.back();
+                    q.pos[bin[op].second].second = op;
+                }
+                bin.pop_back();
+            }
+            q.pos[idx].first = nb;
+            if (nb >= 0) {
+                q.table[nb].push_back({ft, idx});
+                q.pos[idx].second = (int)q.table[nb].size() - 1;
+            }
+        } else if (nb >= 0) {
+            q.table[nb][q.pos[idx].second].first = ft;
+        }
+    }
+    /* NRMConstant_v5::rebuild (NRMConstant_v5.cpp:194-242) */
+    bool q_rebuild() {
+        double est;
+        if (q.rxnCount > 0) est = (double)q.rxnCount / (q.U - q.L);
+        else est = 1.0 / ((q.U - q.L) / (double)q.nbins);
+        if (!set_bins(q.U, est, (int)q.active)) { q.minBin = 0; return false; }
+        q.minBin = 0;
+        insert_all();
+        int nz = 0;
+        for (auto &b : q.table) nz += (int)b.size();
+        q.rxnCount = 0;
+        return nz > 0;
+    }
+    /* NRMConstant_v5::selectReaction (NRMConstant_v5.cpp:156-178) */
+    std::pair<double, int> q_select() {
+        if (q.table.empty()) return {std::numeric_limits<double>::max(), -1};
+        while (q.table[q.minBin].empty()) {
+            ++q.minBin;
+            if (q.minBin == q.table.size()) {
+                if (!q_rebuild()) return {std::numeric_limits<double>::max(), -1};
+                if (q.table.empty()) return {std::numeric_limits<double>::max(), -1};
+            }
+        }
+        auto &bin = q.table[q.minBin];
+        size_t m = 0;
+        for (size_t i = 1; i < bin.size(); i++)
+            if (bin[i].first < bin[m].first) m = i;
+        ++q.rxnCount;
+        return bin[m];
+    }
+    /* nsm_core__initialize_heap (simulate_rdme.cpp:155-195) */
+    bool init_heap() {
+        std::vector<double> prop(n);
+        double psum = 0.0;
+        size_t act = 0;
+        for (int i = 0; i < n; i++) {
+            prop[i] = srrate[i] + sdrate[i];
+            psum += prop[i];
+            if (prop[i] > 0) act++;
+        }
+        double t0 = current_step * dt;
+        double tend = t0 + dt;
+        if (static_domain) tend = nt * dt;
+        if (S > 0 && psum > 0.0) return q_build(prop, psum, act, t0, tend);
+        return false;
+    }
+
+    void record(double t, int vox, int kind, int which, int dest) {
+        if (trace_n < trace_cap) trace.push_back({t, vox, kind, which, dest});
+        trace_n++;
+    }
+
+    /* nsm_core__take_step (simulate_rdme.cpp:211-472) */
+    int take_rdme_step(double t_now, double step_size) {
+        if (S == 0) return 0;
+        double tt = t_now, end_time = t_now + step_size;
+        int dest = -1;  /* persists across events of one call (simulate_rdme.cpp:223) */
+        const bool textbook = flags & SSASDPD_FLAG_TEXTBOOK_SELECTION;
+        while (tt <= end_time) {
+            auto pr = q_select();
+            tt = pr.first;
+            int sv = pr.second;
+            if (sv == -1) return 0;
+            double vol = mass[sv] / rho[sv];
+            double totrate = srrate[sv] + sdrate[sv];
+            double rand1 = rng.uniform();
+            if (rng.exhausted) return SSASDPD_ERR_REPLAY_EXHAUSTED;
+            uint32_t *xs = &xx[(size_t)sv * S];
+            double *rr = &rrate[(size_t)sv * R];
+            double *Dd = &Ddiag[(size_t)sv * S];
+            int event;
+            if (rand1 <= srrate[sv] / totrate) {
+                event = 0;
+                double rv = textbook ? rand1 * totrate : rand1 * srrate[sv];
+                int re;
+                double cum;
+                /* the reference reads rrate[R] when the search runs off the
+                 * end (simulate_rdme.cpp:261); that element counts as 0 here */
+                for (re = 0, cum = rr[0]; re < R && rv > cum; re++, cum += (re < R ? rr[re] : 0.0)) {}
+                if (re >= R) {
+                    if (cum != srrate[sv]) {
+                        double rd = 0.0;
+                        for (int j = 0; j < R; j++) rd += (rr[j] = propensity(j, xs, vol, type[sv]));
+                        srrate[sv] = rd;
+                    }
+                    if (srrate[sv] == 0.0) continue;
+                    double rv2 = rand1 * srrate[sv];
+                    for (re = 0, cum = rr[0]; re < R && rv2 > cum; re++, cum += (re < R ? rr[re] : 0.0)) {}
+                    if (re >= R) return SSASDPD_ERR_PROPENSITY_OVERFLOW;
+                }
+                for (int k = jcN[re]; k < jcN[re + 1]; k++) {
+                    xs[irN[k]] += (uint32_t)prN[k];
+                    sdrate[sv] += Dd[irN[k]] * prN[k];
+                }
+                double rd = 0.0;
+                for (int k = jcG[S + re]; k < jcG[S + re + 1]; k++) {
+                    int j = irG[k];
+                    double old = rr[j];
+                    rd += (rr[j] = propensity(j, xs, vol, type[sv])) - old;
+                }
+                srrate[sv] += rd;
+                total_reactions++;
+                record(tt, sv, 0, re, -1);
+            } else {
+                event = 1;
+                double drand = textbook ? (rand1 * totrate - srrate[sv]) : rand1 * sdrate[sv];
+                int spec;
+                double cum;
+                for (spec = 0, cum = Dd[0] * xs[0]; spec < S && drand > cum;
+                     spec++, cum += (spec < S ? Dd[spec] * xs[spec] : 0.0)) {}
+                if (spec >= S) {
+                    if (cum != sdrate[sv]) {
+                        double rd = 0.0;
+                        for (int j = 0; j < S; j++) rd += Dd[j] * xs[j];
+                        sdrate[sv] = rd;
+                    }
+                    if (sdrate[sv] == 0.0) continue;
+                    drand = cum * rand1;
+                    for (spec = 0, cum = Dd[0] * xs[0]; spec < S && drand > cum;
+                         spec++, cum += (spec < S ? Dd[spec] * xs[spec] : 0.0)) {}
+                    if (spec >= S) {
+                        spec--;
+                        while (xs[spec] <= 0) {
+                            spec--;
+                            if (spec <= 0) return SSASDPD_ERR_DIFFUSION_OVERFLOW;
+                        }
+                    }
+                }
+                double r2 = rng.uniform();
+                if (rng.exhausted) return SSASDPD_ERR_REPLAY_EXHAUSTED;
+                double rand2 = r2 * Dd[spec];
+                double cum2 = 0.0;
+                bool found = false;
+                for (const Nbr &nb : nbrs[sv]) {
+                    double dc = Dmat[(size_t)spec * T + (type[nb.j] - 1)];
+                    cum2 += nb.Dij * dc;
+                    if (cum2 > rand2) { dest = nb.j; found = true; break; }
+                }
+                if (!found && dest < 0) {
+                    rand2 = r2 * cum2;
+                    cum2 = 0.0;
+                    for (const Nbr &nb : nbrs[sv]) {
+                        double dc = Dmat[(size_t)spec * T + (type[nb.j] - 1)];
+                        cum2 += nb.Dij * dc;
+                        if (cum2 > rand2) { dest = nb.j; found = true; break; }
+                    }
+                    if (!found) return SSASDPD_ERR_DIFFUSION_OVERFLOW;
+                }
+                xs[spec]--;
+                xx[(size_t)dest * S + spec]++;
+                if (R > 0) {
+                    double rd = 0.0, rrd = 0.0;
+                    uint32_t *xd = &xx[(size_t)dest * S];
+                    double *rrdst = &rrate[(size_t)dest * R];
+                    for (int k = jcG[spec]; k < jcG[spec + 1]; k++) {
+                        int j = irG[k];
+                        double old = rr[j];
+                        rd += (rr[j] = propensity(j, xs, vol, type[sv])) - old;
+                        old = rrdst[j];
+                        rrd += (rrdst[j] = propensity(j, xd, vol, type[dest])) - old;
+                    }
+                    srrate[sv] += rd;
+                    srrate[dest] += rrd;
+                }
+                sdrate[sv] -= Dd[spec];
+                sdrate[dest] += Ddiag[(size_t)dest * S + spec];
+                total_diffusion++;
+                record(tt, sv, 1, spec, dest);
+            }
+            totrate = srrate[sv] + sdrate[sv];
+            q_update(sv, totrate, tt);
+            if (event) {
+                totrate = srrate[dest] + sdrate[dest];
+                q_update(dest, totrate, tt);
+            }
+            if (rng.exhausted) return SSASDPD_ERR_REPLAY_EXHAUSTED;
+        }
+        return 0;
+    }
+
+    /* simulate_rdme (simulate_rdme.cpp:53-68) */
+    int simulate_rdme(uint32_t step) {
+        if (flags & SSASDPD_FLAG_NO_RDME) return 0;
+        if (!static_domain || !initialized) {
+            init_rxn();
+            if (!init_diff()) return err;
+            if (init_heap()) initialized = 1;
+            else return rng.exhausted ? SSASDPD_ERR_REPLAY_EXHAUSTED : 0;
+        }
+        return take_rdme_step(dt * step, dt);
+    }
+
+    /* One iteration of the step loop (simulate_threads.cpp:232-281; the
+     * worker body run_simulation_thread :130-150 with one thread). */
+    int do_step() {
+        uint32_t step = current_step;
+        build_tree();
+        /* substep 0 */
+        for (int i = 0; i < n; i++)
+            if (!nan_check(i)) return err;
+        for (int i = 0; i < n; i++) {
+            if (step == 0 || static_domain == 0)
+                if (!find_neighbors(i)) return err;
+            step1(i, step);
+        }
+        /* substep 1 */
+        for (int i = 0; i < n; i++) {
+            if (step > 0 && static_domain == 0)
+                if (!find_neighbors(i)) return err;
+            pairwise_force(i);
+        }
+        /* substep 2 */
+        if (gauss_seidel) {
+            for (int i = 0; i < n; i++) { step2a(i, step); step2b(i, rho); }
+        } else {
+            for (int i = 0; i < n; i++) step2a(i, step);
+            std::vector<double> snap = rho;
+            for (int i = 0; i < n; i++) step2b(i, snap);
+        }
+        int rc = simulate_rdme(step);
+        current_step++;
+        return rc;
+    }
+
+    void reset() {
+        x = x0;
+        v = v0;
+        vt.assign(3 * (size_t)n, 0.0);
+        F.assign(3 * (size_t)n, 0.0);
+        Fbp.assign(3 * (size_t)n, 0.0);
+        Frho.assign(n, 0.0);
+        rho = rho_init;
+        old_rho = rho;
+        bvfphi.assign(n, 0.0);
+        normal.assign(3 * (size_t)n, 0.0);
+        C = C_init;
+        Q.assign((size_t)n * Sc, 0.0);
+        nbrs.assign(n, {});
+        xx = u0;
+        rrate.assign((size_t)n * R, 0.0);
+        srrate.assign(n, 0.0);
+        sdrate.assign(n, 0.0);
+        Ddiag.assign((size_t)n * S, 0.0);
+        initialized = 0;
+        total_reactions = total_diffusion = 0;
+        current_step = 0;
+        tree_built = false;
+        err = 0;
+        bad_particle = -1;
+        trace.clear();
+        trace_n = 0;
+        q = Queue();
+    }
+};
+
+static int validate(const ssasdpd_desc *d) {
+    if (!d) return fail(SSASDPD_ERR_BAD_DESC, "null descriptor");
+    if (d->n <= 0 || d->dimension < 1 || d->dimension > 3 || d->n_types < 1)
+        return fail(SSASDPD_ERR_BAD_DESC, "bad sizes");
+    if (!d->x || !d->type || !d->mass || !d->rho || !d->nu || !d->fixed)
+        return fail(SSASDPD_ERR_BAD_DESC, "missing particle arrays");
+    if (d->n_species > 0 && (!d->u0 || !d->diffusion_matrix || !d->jcG || !d->irG))
+        return fail(SSASDPD_ERR_BAD_DESC, "missing chemistry arrays");
+    if (d->n_reactions > 0 && (!d->reactions || !d->jcN || !d->irN || !d->prN))
+        return fail(SSASDPD_ERR_BAD_DESC, "missing reaction arrays");
+    return 0;
+}
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+void *oracle_create(const ssasdpd_desc *d, int gauss_seidel) {
+    if (validate(d)) return nullptr;
+    Sim *s = new Sim();
+    int n = d->n;
+    s->n = n; s->dim = d->dimension; s->T = d->n_types;
+    s->S = d->n_species; s->R = d->n_reactions;
+    s->Sc = d->n_chem_species; s->Rc = d->n_chem_reactions;
+    s->static_domain = d->static_domain; s->nt = d->nt; s->flags = d->flags;
+    s->dt = d->dt; s->h = d->h; s->rho0 = d->rho0; s->c0 = d->c0; s->P0 = d->P0;
+    for (int k = 0; k < 3; k++) s->grav[k] = d->gravity[k];
+    s->gauss_seidel = gauss_seidel;
+    s->x0.assign(d->x, d->x + 3 * (size_t)n);
+    s->v0.assign(3 * (size_t)n, 0.0);
+    if (d->v) s->v0.assign(d->v, d->v + 3 * (size_t)n);
+    s->type.assign(d->type, d->type + n);
+    s->solid.assign(d->fixed, d->fixed + n);
+    s->mass.assign(d->mass, d->mass + n);
+    s->nu.assign(d->nu, d->nu + n);
+    s->rho_init.assign(d->rho, d->rho + n);
+    int S = d->n_species, R = d->n_reactions, T = d->n_types;
+    if (S > 0) {
+        s->u0.assign(d->u0, d->u0 + (size_t)n * S);
+        s->Dmat.assign(d->diffusion_matrix, d->diffusion_matrix + (size_t)S * T);
+        s->jcG.assign(d->jcG, d->jcG + S + R + 1);
+        s->irG.assign(d->irG, d->irG + s->jcG.back());
+    } else if (d->n_chem_species > 0 && d->diffusion_matrix) {
+        s->Dmat.assign(d->diffusion_matrix, d->diffusion_matrix + (size_t)d->n_chem_species * T);
+    }
+    if (R > 0) {
+        s->rxn.assign(d->reactions, d->reactions + R);
+        s->jcN.assign(d->jcN, d->jcN + R + 1);
+        s->irN.assign(d->irN, d->irN + s->jcN.back());
+        s->prN.assign(d->prN, d->prN + s->jcN.back());
+    } else if (d->n_chem_reactions > 0 && d->reactions) {
+        s->rxn.assign(d->reactions, d->reactions + d->n_chem_reactions);
+    }
+    if (d->N_dense && d->n_chem_reactions > 0 && d->n_chem_species > 0)
+        s->Ndense.assign(d->N_dense, d->N_dense + (size_t)d->n_chem_species * d->n_chem_reactions);
+    int Sc = d->n_chem_species;
+    s->C_init.assign((size_t)n * Sc, 0.0);
+    for (size_t k = 0; k < (size_t)n * Sc; k++)
+        s->C_init[k] = d->C0 ? d->C0[k] : (S == Sc && d->u0 ? (double)d->u0[k] : 0.0);
+    if (d->n_bcs > 0) s->bcs.assign(d->bcs, d->bcs + d->n_bcs);
+    if (d->n_output_steps > 0) s->out_steps.assign(d->output_steps, d->output_steps + d->n_output_steps);
+    s->reset();
+    return s;
+}
+
+void oracle_destroy(void *p) { delete (Sim *)p; }
+
+int oracle_reset(void *p, const ssasdpd_rng *r) {
+    Sim *s = (Sim *)p;
+    s->reset();
+    s->rng = Stream();
+    if (!r || r->kind == 0) {
+        s->rng.kind = 0;
+        s->rng.mt.seed_with(r ? r->seed : 5489ULL);
+    } else {
+        s->rng.kind = 1;
+        s->rng.words = r->words;
+        s->rng.expo = r->expo;
+        s->rng.n = r->n_words;
+    }
+    return 0;
+}
+
+int oracle_advance(void *p, uint32_t nsteps) {
+    Sim *s = (Sim *)p;
+    for (uint32_t k = 0; k < nsteps && s->current_step < s->nt; k++) {
+        int rc = s->do_step();
+        if (rc) return fail(rc, "oracle step failed");
+    }
+    return 0;
+}
+
+static void snapshot(Sim *s, ssasdpd_outputs *o, int k) {
+    size_t n = s->n;
+    if (o->xx && s->S) memcpy(o->xx + k * n * s->S, s->xx.data(), n * s->S * 4);
+    if (o->C && s->Sc) memcpy(o->C + k * n * s->Sc, s->C.data(), n * s->Sc * 8);
+    if (o->x) memcpy(o->x + k * n * 3, s->x.data(), n * 3 * 8);
+    if (o->v) memcpy(o->v + k * n * 3, s->v.data(), n * 3 * 8);
+    if (o->rho) memcpy(o->rho + k * n, s->rho.data(), n * 8);
+    if (o->bvf_phi) memcpy(o->bvf_phi + k * n, s->bvfphi.data(), n * 8);
+    if (o->nu) memcpy(o->nu + k * n, s->nu.data(), n * 8);
+    if (o->steps) o->steps[k] = s->current_step;
+}
+
+/* run_simulation's output schedule (simulate_threads.cpp:231-289) with the
+ * generated get_next_output (solver.py:290-299). */
+int oracle_run(void *p, const ssasdpd_rng *r, ssasdpd_outputs *o) {
+    Sim *s = (Sim *)p;
+    oracle_reset(p, r);
+    uint32_t next = 0;
+    size_t oidx = 0;
+    int k = 0;
+    for (uint32_t step = 0; step < s->nt; step++) {
+        if (step >= next) {
+            if (k < o->n_snapshots) snapshot(s, o, k);
+            k++;
+            next = oidx < s->out_steps.size() ? s->out_steps[oidx] : 0xFFFFFFFFu;
+            oidx++;
+        }
+        int rc = s->do_step();
+        if (rc) return fail(rc, "oracle step failed");
+    }
+    if (k < o->n_snapshots) snapshot(s, o, k);
+    k++;
+    return 0;
+}
+
+int oracle_stats(void *p, ssasdpd_stats *st) {
+    Sim *s = (Sim *)p;
+    memset(st, 0, sizeof(*st));
+    st->steps = s->current_step;
+    st->particle_steps = (uint64_t)s->current_step * s->n;
+    st->reactions = s->total_reactions;
+    st->diffusions = s->total_diffusion;
+    st->rng_words_used = s->rng.pos;
+    st->bad_particle = s->bad_particle;
+    int md = 0;
+    for (auto &v : s->nbrs) md = std::max(md, (int)v.size());
+    st->max_degree = md;
+    return 0;
+}
+
+int oracle_get_state(void *p, ssasdpd_state *st) {
+    Sim *s = (Sim *)p;
+    size_t n = s->n;
+    auto cp = [](double *dst, const std::vector<double> &src) { if (dst && !src.empty()) memcpy(dst, src.data(), src.size() * 8); };
+    cp(st->x, s->x); cp(st->v, s->v); cp(st->vt, s->vt); cp(st->F, s->F); cp(st->Fbp, s->Fbp);
+    cp(st->rho, s->rho); cp(st->Frho, s->Frho); cp(st->bvf_phi, s->bvfphi); cp(st->nu, s->nu);
+    cp(st->old_rho, s->old_rho); cp(st->C, s->C); cp(st->Q, s->Q);
+    cp(st->srrate, s->srrate); cp(st->sdrate, s->sdrate); cp(st->rrate, s->rrate); cp(st->Ddiag, s->Ddiag);
+    if (st->xx && !s->xx.empty()) memcpy(st->xx, s->xx.data(), s->xx.size() * 4);
+    if (st->next_time) {
+        for (size_t i = 0; i < n; i++)
+            st->next_time[i] = s->q.next.empty() ? std::numeric_limits<double>::infinity() : s->q.next[i];
+    }
+    return 0;
+}
+
+int oracle_get_neighbors(void *p, int64_t *row_ptr, int32_t *idx, double *dist, double *dWdr, double *Dij) {
+    Sim *s = (Sim *)p;
+    int64_t off = 0;
+    for (int i = 0; i < s->n; i++) {
+        row_ptr[i] = off;
+        for (auto &nb : s->nbrs[i]) {
+            if (idx) idx[off] = nb.j;
+            if (dist) dist[off] = nb.dist;
+            if (dWdr) dWdr[off] = nb.dWdr;
+            if (Dij) Dij[off] = nb.Dij;
+            off++;
+        }
+    }
+    row_ptr[s->n] = off;
+    return 0;
+}
+
+int oracle_set_trace_capacity(void *p, uint64_t cap) { ((Sim *)p)->trace_cap = cap; return 0; }
+int oracle_get_trace(void *p, ssasdpd_event *ev, uint64_t cap, uint64_t *nev) {
+    Sim *s = (Sim *)p;
+    uint64_t m = std::min<uint64_t>(cap, s->trace.size());
+    if (ev) memcpy(ev, s->trace.data(), m * sizeof(ssasdpd_event));
+    *nev = s->trace_n;
+    return 0;
+}
+
+/* Fixed-radius search through the kd-tree restatement. */
+int oracle_fr_search(const double *pts, int32_t n, const double *q, int32_t m, int32_t dim, double h,
+                     int64_t *row_ptr, int32_t *idx, double *r2) {
+    KdTree t;
+    std::vector<double> p((size_t)n * dim), qq(3);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) p[(size_t)i * dim + d] = pts[3 * (size_t)i + d];
+    t.build(p.data(), n, dim);
+    std::vector<KdTree::Hit> hits;
+    int64_t off = 0;
+    for (int i = 0; i < m; i++) {
+        for (int d = 0; d < dim; d++) qq[d] = q[3 * (size_t)i + d];
+        t.fr_search(qq.data(), h * h, hits);
+        row_ptr[i] = off;
+        for (auto &hh : hits) {
+            if (idx) { idx[off] = hh.i; r2[off] = hh.d; }
+            off++;
+        }
+    }
+    row_ptr[m] = off;
+    return 0;
+}
+
+/* Brute force over all pairs: the acceptance rule only (r2 <= h*h, r2 != 0,
+ * sqrt(r2) <= h), ordered by (r2, index).  Pins the neighbour SET and the
+ * r2 bits independently of any tree. */
+int oracle_fr_brute(const double *pts, int32_t n, const double *q, int32_t m, int32_t dim, double h,
+                    int64_t *row_ptr, int32_t *idx, double *r2) {
+    int64_t off = 0;
+    double sq = h * h;
+    std::vector<std::pair<double, int>> row;
+    for (int i = 0; i < m; i++) {
+        row.clear();
+        for (int j = 0; j < n; j++) {
+            double dist = 0;
+            for (int d = 0; d < dim; d++) {
+                double t = q[3 * (size_t)i + d] - pts[3 * (size_t)j + d];
+                dist = dist + t * t;
+            }
+            if (dist <= sq && dist != 0 && std::sqrt(dist) <= h) row.push_back({dist, j});
+        }
+        std::sort(row.begin(), row.end());
+        row_ptr[i] = off;
+        for (auto &e : row) {
+            if (idx) { idx[off] = e.second; r2[off] = e.first; }
+            off++;
+        }
+    }
+    row_ptr[m] = off;
+    return 0;
+}
+
+/* kd-tree structure dump for the ANN golden files: number of nodes of each
+ * kind (leaves, empty leaves, splits) and the tree depth. */
+int oracle_tree_stats(const double *pts, int32_t n, int32_t dim, int32_t *n_split, int32_t *n_leaf,
+                      int32_t *n_empty, int32_t *depth) {
+    KdTree t;
+    std::vector<double> p((size_t)n * dim);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) p[(size_t)i * dim + d] = pts[3 * (size_t)i + d];
+    t.build(p.data(), n, dim);
+    int ns = 0, nl = 0, ne = 0;
+    for (auto &nd : t.nodes) {
+        if (!nd.leaf) ns++;
+        else if (nd.pt < 0) ne++;
+        else nl++;
+    }
+    std::vector<std::pair<int, int>> st;
+    int md = 0;
+    if (t.root >= 0) st.push_back({t.root, 0});
+    while (!st.empty()) {
+        auto [ni, dd] = st.back();
+        st.pop_back();
+        md = std::max(md, dd);
+        if (!t.nodes[ni].leaf) {
+            st.push_back({t.nodes[ni].child[0], dd + 1});
+            st.push_back({t.nodes[ni].child[1], dd + 1});
+        }
+    }
+    *n_split = ns; *n_leaf = nl; *n_empty = ne; *depth = md;
+    return 0;
+}
+
+int oracle_mt19937_64_stream(uint64_t seed, uint64_t n, uint64_t *words, double *expo) {
+    MT64 mt(seed);
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t w = mt.next();
+        if (words) words[i] = w;
+        if (expo) expo[i] = word_to_expo(w);
+    }
+    return 0;
+}
+
+}  // extern "C"

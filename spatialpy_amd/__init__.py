@@ -1,0 +1,10 @@
+"""spatialpy_amd -- SpatialPy's sSSA-SDPD engine rebuilt for AMD MI355X (gfx950).
+
+The package keeps SpatialPy's Model/Solver interface and runs the engine as
+hand-written HIP kernels in libssasdpd.so, called through a C ABI
+(include/ssasdpd.h).
+"""
+from .core.errors import ModelError, DomainError, SimulationError, SimulationTimeout
+from .lowering import SystemArrays, output_schedule
+
+__version__ = "0.1.0"
