@@ -1,0 +1,826 @@
+// Host side of the engine: device memory, the step loop and the C ABI
+// (include/ssasdpd.h).
+//
+// One engine = one model on one GPU.  Work is issued on one HIP stream; the
+// step loop follows run_simulation (ENG/src/simulate_threads.cpp:171-312)
+// with the three worker substeps (ENG/src/simulate.cpp:33-173) and the serial
+// RDME step (ENG/src/simulate_rdme.cpp:53-68) as kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace ssa;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class T> T *dalloc(size_t count) {
+    T *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_OK(hipMalloc((void **)&p, sizeof(T) * count));
+    return p;
+}
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+template <class T> void h2d(hipStream_t s, T *dst, const T *src, size_t count) {
+    if (count) HIP_OK(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+template <class T> void d2h(hipStream_t s, T *dst, const T *src, size_t count) {
+    if (count) HIP_OK(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+}
+
+// component-major [3][n] <-> particle-major [n][3]
+std::vector<double> to_soa3(const double *aos, int n) {
+    std::vector<double> o(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) o[(size_t)k * n + i] = aos ? aos[3 * (size_t)i + k] : 0.0;
+    return o;
+}
+void from_soa3(const std::vector<double> &soa, double *aos, int n) {
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) aos[3 * (size_t)i + k] = soa[(size_t)k * n + i];
+}
+
+double kernel_alpha(int dim, double h) {
+    // ENG/src/particle.cpp:169-175 (1-D: 5 / 4 * h is integer 1 times h)
+    if (dim == 3) return 105 / (16 * M_PI * h * h * h);
+    if (dim == 2) return 5 / (M_PI * h * h);
+    return 5 / 4 * h;
+}
+
+}  // namespace
+
+struct ssasdpd_engine {
+    // configuration
+    int n = 0, dim = 3, T = 1, S = 0, R = 0, Sc = 0, Rc = 0, static_domain = 1, nbc = 0;
+    uint32_t nt = 0, flags = 0;
+    double dt = 0, h = 0, rho0 = 1, c0 = 10, P0 = 100, alpha = 0;
+    double g[3] = {0, 0, 0};
+    std::vector<uint32_t> out_steps;
+    // initial state (host)
+    std::vector<double> x0, v0, rho0v, C0;
+    std::vector<uint32_t> u0;
+    std::vector<double> nu0;
+
+    hipStream_t st = nullptr;
+    // particles
+    double *x = nullptr, *v = nullptr, *vt = nullptr, *F = nullptr, *Fbp = nullptr, *normal = nullptr;
+    double *rho = nullptr, *old_rho = nullptr, *Frho = nullptr, *mass = nullptr, *nu = nullptr, *bvf = nullptr;
+    int *type = nullptr, *solid = nullptr;
+    double *C = nullptr, *Q = nullptr;
+    // neighbour lists
+    int W = 0;
+    int *nidx = nullptr, *deg = nullptr;
+    double *nr2 = nullptr, *nr = nullptr, *Dij = nullptr;
+    // binning / tree
+    double *xtree = nullptr, *spts = nullptr, *d_bbox = nullptr;
+    unsigned long long *d_bbtmp = nullptr;
+    int *sorted = nullptr, *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
+    long long cell_cap = 0;
+    Grid grid{};
+    KdDev kd;
+    void *kd_work = nullptr;
+    size_t kd_work_sz = 0;
+    void *scan_tmp = nullptr;
+    // chemistry constants
+    Rxn *rx = nullptr;
+    int *jcN = nullptr, *irN = nullptr, *prN = nullptr, *jcG = nullptr, *irG = nullptr, *Ndense = nullptr;
+    double *Dmat = nullptr;
+    BcRule *bc = nullptr;
+    // RDME state
+    uint32_t *xx = nullptr;
+    double *rrate = nullptr, *srrate = nullptr, *sdrate = nullptr, *Ddiag = nullptr, *qtime = nullptr, *tmin = nullptr;
+    int *targ = nullptr;
+    int nlev = 0, lev_off[8] = {0}, lev_n[8] = {0};
+    QueueState *qs = nullptr;
+    int *qwork = nullptr;
+    double *d_psum = nullptr;
+    // random stream
+    int rng_kind = 0;
+    uint64_t seed = 0;
+    uint64_t *d_words = nullptr;
+    double *d_expo = nullptr;
+    uint64_t n_words = 0, words_cap = 0;
+    // trace
+    ssasdpd_event *d_trace = nullptr;
+    uint64_t trace_cap = 0;
+    // errors
+    DevErr *derr = nullptr;
+    // progress
+    uint32_t cur = 0;
+    bool lists_built = false;
+    bool kd_built = false;
+    bool rdme_initialized = false;
+    ssasdpd_stats stats{};
+    int max_deg_seen = 0;
+
+    ~ssasdpd_engine() { release(); }
+
+    void release() {
+        dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
+        dfree(rho); dfree(old_rho); dfree(Frho); dfree(mass); dfree(nu); dfree(bvf);
+        dfree(type); dfree(solid); dfree(C); dfree(Q);
+        dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij);
+        dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
+        dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
+        dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
+        if (kd_work) (void)hipFree(kd_work);
+        kd_work = nullptr;
+        if (scan_tmp) (void)hipFree(scan_tmp);
+        scan_tmp = nullptr;
+        dfree(rx); dfree(jcN); dfree(irN); dfree(prN); dfree(jcG); dfree(irG); dfree(Ndense); dfree(Dmat); dfree(bc);
+        dfree(xx); dfree(rrate); dfree(srrate); dfree(sdrate); dfree(Ddiag); dfree(qtime); dfree(tmin); dfree(targ);
+        dfree(qs); dfree(qwork); dfree(d_psum); dfree(d_words); dfree(d_expo); dfree(d_trace); dfree(derr);
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+
+    void alloc_lists(int w) {
+        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij);
+        W = w;
+        size_t cap = (size_t)W * n;
+        nidx = dalloc<int>(cap);
+        nr2 = dalloc<double>(cap);
+        nr = dalloc<double>(cap);
+        Dij = dalloc<double>(cap);
+    }
+
+    void setup(const ssasdpd_desc *d) {
+        n = d->n; dim = d->dimension; T = d->n_types; S = d->n_species; R = d->n_reactions;
+        Sc = d->n_chem_species; Rc = d->n_chem_reactions; static_domain = d->static_domain;
+        nbc = d->n_bcs; nt = d->nt; flags = d->flags;
+        dt = d->dt; h = d->h; rho0 = d->rho0; c0 = d->c0; P0 = d->P0;
+        for (int k = 0; k < 3; k++) g[k] = d->gravity[k];
+        alpha = kernel_alpha(dim, h);
+        out_steps.assign(d->output_steps, d->output_steps + d->n_output_steps);
+        HIP_OK(hipStreamCreate(&st));
+        size_t n3 = 3 * (size_t)n;
+        x = dalloc<double>(n3); v = dalloc<double>(n3); vt = dalloc<double>(n3); F = dalloc<double>(n3);
+        Fbp = dalloc<double>(n3); normal = dalloc<double>(n3); xtree = dalloc<double>(n3); spts = dalloc<double>(n3);
+        rho = dalloc<double>(n); old_rho = dalloc<double>(n); Frho = dalloc<double>(n); mass = dalloc<double>(n);
+        nu = dalloc<double>(n); bvf = dalloc<double>(n);
+        type = dalloc<int>(n); solid = dalloc<int>(n);
+        C = dalloc<double>((size_t)n * Sc); Q = dalloc<double>((size_t)n * Sc);
+        deg = dalloc<int>(n);
+        sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
+        d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
+        derr = dalloc<DevErr>(1);
+        d_psum = dalloc<double>(2);
+        kd.parent = dalloc<int>(2 * (size_t)n + 2); kd.depth = dalloc<int>(2 * (size_t)n + 2);
+        kd.cd = dalloc<int>(2 * (size_t)n + 2); kd.cv = dalloc<double>(2 * (size_t)n + 2);
+        kd.is_hi = dalloc<int>(2 * (size_t)n + 2); kd.leaf_of = dalloc<int>(n);
+        kd_work_sz = kd_work_bytes(n);
+        HIP_OK(hipMalloc(&kd_work, kd_work_sz));
+        size_t scan_sz = std::max(scan_tmp_bytes(4LL * n + (1 << 20) + 8), scan_tmp_bytes(n + 8));
+        HIP_OK(hipMalloc(&scan_tmp, scan_sz));
+        alloc_lists(dim == 3 ? 40 : (dim == 2 ? 16 : 4));
+
+        // constants
+        std::vector<int> ty(d->type, d->type + n), so(d->fixed, d->fixed + n);
+        h2d(st, type, ty.data(), n);
+        h2d(st, solid, so.data(), n);
+        h2d(st, mass, d->mass, n);
+        x0 = to_soa3(d->x, n);
+        v0 = to_soa3(d->v, n);
+        rho0v.assign(d->rho, d->rho + n);
+        nu0.assign(d->nu, d->nu + n);
+        if (S > 0) u0.assign(d->u0, d->u0 + (size_t)n * S);
+        C0.assign((size_t)n * Sc, 0.0);
+        for (size_t k = 0; k < (size_t)n * Sc; k++)
+            C0[k] = d->C0 ? d->C0[k] : (S == Sc && d->u0 ? (double)d->u0[k] : 0.0);
+        int Smax = std::max(S, Sc), Rmax = std::max(R, Rc);
+        if (Smax > 0 && d->diffusion_matrix) {
+            Dmat = dalloc<double>((size_t)Smax * T);
+            h2d(st, Dmat, d->diffusion_matrix, (size_t)Smax * T);
+        }
+        if (Rmax > 0) {
+            std::vector<Rxn> r(Rmax);
+            for (int i = 0; i < Rmax; i++) {
+                const ssasdpd_reaction &q = d->reactions[i];
+                r[i] = Rxn{q.kind, q.a, q.b, 0, q.k, q.type_mask};
+            }
+            rx = dalloc<Rxn>(Rmax);
+            h2d(st, rx, r.data(), Rmax);
+        }
+        if (R > 0) {
+            int nnz = d->jcN[R];
+            jcN = dalloc<int>(R + 1); h2d(st, jcN, d->jcN, R + 1);
+            irN = dalloc<int>(nnz); h2d(st, irN, d->irN, nnz);
+            prN = dalloc<int>(nnz); h2d(st, prN, d->prN, nnz);
+        }
+        if (S > 0) {
+            int nnz = d->jcG[S + R];
+            jcG = dalloc<int>(S + R + 1); h2d(st, jcG, d->jcG, S + R + 1);
+            irG = dalloc<int>(nnz); h2d(st, irG, d->irG, nnz);
+            xx = dalloc<uint32_t>((size_t)n * S);
+            rrate = dalloc<double>((size_t)n * std::max(R, 1));
+            srrate = dalloc<double>(n); sdrate = dalloc<double>(n);
+            Ddiag = dalloc<double>((size_t)n * S);
+            qtime = dalloc<double>(n);
+            // 64-ary min-tree levels
+            lev_n[0] = n;
+            nlev = 0;
+            int tot = 0;
+            while (lev_n[nlev] > 1) {
+                lev_off[nlev] = tot;
+                lev_n[nlev + 1] = (lev_n[nlev] + TREE_FAN - 1) / TREE_FAN;
+                tot += lev_n[nlev + 1];
+                nlev++;
+                if (nlev >= 7) throw HipError("too many voxels for the event tree", hipErrorUnknown, __FILE__, __LINE__);
+            }
+            tmin = dalloc<double>(std::max(tot, 1));
+            targ = dalloc<int>(std::max(tot, 1));
+            qs = dalloc<QueueState>(1);
+            qwork = (int *)dalloc<char>(sizeof(double) * (size_t)n + sizeof(int) * 4 * ((size_t)n + 1) + 64);
+        }
+        if (Sc > 0 && Rc > 0 && d->N_dense) {
+            Ndense = dalloc<int>((size_t)Sc * Rc);
+            h2d(st, Ndense, d->N_dense, (size_t)Sc * Rc);
+        }
+        if (nbc > 0) {
+            std::vector<BcRule> b(nbc);
+            for (int i = 0; i < nbc; i++) {
+                const ssasdpd_bc &q = d->bcs[i];
+                for (int k = 0; k < 3; k++) { b[i].lo[k] = q.lo[k]; b[i].hi[k] = q.hi[k]; b[i].value[k] = q.value[k]; }
+                b[i].type = q.type; b[i].target = q.target; b[i].species = q.species; b[i].pad = 0;
+            }
+            bc = dalloc<BcRule>(nbc);
+            h2d(st, bc, b.data(), nbc);
+        }
+        HIP_OK(hipStreamSynchronize(st));
+    }
+
+    void reset(const ssasdpd_rng *r) {
+        size_t n3 = 3 * (size_t)n;
+        h2d(st, x, x0.data(), n3);
+        h2d(st, v, v0.data(), n3);
+        HIP_OK(hipMemsetAsync(vt, 0, sizeof(double) * n3, st));
+        HIP_OK(hipMemsetAsync(F, 0, sizeof(double) * n3, st));
+        HIP_OK(hipMemsetAsync(Fbp, 0, sizeof(double) * n3, st));
+        HIP_OK(hipMemsetAsync(normal, 0, sizeof(double) * n3, st));
+        h2d(st, rho, rho0v.data(), n);
+        h2d(st, old_rho, rho0v.data(), n);
+        h2d(st, nu, nu0.data(), n);
+        HIP_OK(hipMemsetAsync(Frho, 0, sizeof(double) * n, st));
+        HIP_OK(hipMemsetAsync(bvf, 0, sizeof(double) * n, st));
+        if (Sc > 0) {
+            h2d(st, C, C0.data(), (size_t)n * Sc);
+            HIP_OK(hipMemsetAsync(Q, 0, sizeof(double) * n * Sc, st));
+        }
+        HIP_OK(hipMemsetAsync(deg, 0, sizeof(int) * n, st));
+        if (S > 0) {
+            h2d(st, xx, u0.data(), (size_t)n * S);
+            HIP_OK(hipMemsetAsync(srrate, 0, sizeof(double) * n, st));
+            HIP_OK(hipMemsetAsync(sdrate, 0, sizeof(double) * n, st));
+            QueueState q{};
+            h2d(st, qs, &q, 1);
+        }
+        DevErr e0{0, INT32_MAX, 0, 0, 0};
+        h2d(st, derr, &e0, 1);
+        rng_kind = (r && r->kind == 1) ? 1 : 0;
+        seed = r ? r->seed : 0;
+        if (rng_kind == 1) {
+            if (r->n_words > words_cap) {
+                dfree(d_words); dfree(d_expo);
+                words_cap = r->n_words;
+                d_words = dalloc<uint64_t>(words_cap);
+                d_expo = dalloc<double>(words_cap);
+            }
+            n_words = r->n_words;
+            h2d(st, d_words, r->words, n_words);
+            h2d(st, d_expo, r->expo, n_words);
+        } else {
+            n_words = 0;
+        }
+        HIP_OK(hipStreamSynchronize(st));
+        cur = 0;
+        lists_built = false;
+        kd_built = false;
+        rdme_initialized = false;
+        max_deg_seen = 0;
+        memset(&stats, 0, sizeof(stats));
+        stats.bad_particle = -1;
+    }
+
+    DevErr read_err() {
+        DevErr e;
+        d2h(st, &e, derr, 1);
+        HIP_OK(hipStreamSynchronize(st));
+        return e;
+    }
+
+    // Bin the tree points (= current x), and build the kd-tree when list
+    // order needs it.
+    void bin_tree_points() {
+        HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+        compute_bbox(st, xtree, n, dim, d_bbox, d_bbtmp);
+        double bb[6];
+        d2h(st, bb, d_bbox, 6);
+        HIP_OK(hipStreamSynchronize(st));
+        grid_setup(bb, n, dim, h, grid);
+        if (grid.ncell + 1 > cell_cap) {
+            dfree(cell_count); dfree(cell_start); dfree(cell_fill);
+            cell_cap = grid.ncell + 1;
+            cell_count = dalloc<int>(cell_cap);
+            cell_start = dalloc<int>(cell_cap + 1);
+            cell_fill = dalloc<int>(cell_cap);
+        }
+        bin_points(st, xtree, n, dim, grid, cell_of, cell_count, cell_start, cell_fill, sorted, spts, scan_tmp);
+        bool want_kd = static_domain || (flags & SSASDPD_FLAG_ANN_ORDER_MOVING);
+        if (want_kd && !(static_domain && kd_built)) {
+            kd_build(st, xtree, n, dim, kd, kd_work, kd_work_sz);
+            kd_built = true;
+        }
+        if (!want_kd) kd.valid = false;
+    }
+
+    // Fixed-radius search from query = current x against the binned tree
+    // points, then r and D_ij with the current rho (find_neighbors).
+    void search() {
+        for (int attempt = 0; attempt < 4; attempt++) {
+            HIP_OK(hipMemsetAsync(&derr->overflow, 0, sizeof(int), st));
+            HIP_OK(hipMemsetAsync(&derr->max_deg, 0, sizeof(int), st));
+            fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
+                      deg, derr, sorted);
+            DevErr e = read_err();
+            max_deg_seen = std::max(max_deg_seen, e.max_deg);
+            if (!e.overflow) break;
+            alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
+        }
+        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr);
+        lists_built = true;
+    }
+
+    SdpdArgs sdpd_args(unsigned step) {
+        SdpdArgs a{};
+        a.n = n; a.dim = dim; a.Sc = Sc; a.Rc = Rc; a.T = T; a.static_domain = static_domain; a.nbc = nbc; a.W = W;
+        a.step = step; a.dt = dt; a.h = h; a.rho0 = rho0; a.c0 = c0; a.P0 = P0; a.alpha = alpha;
+        for (int k = 0; k < 3; k++) a.g[k] = g[k];
+        a.x = x; a.v = v; a.vt = vt; a.F = F; a.Fbp = Fbp; a.rho = rho; a.old_rho = old_rho; a.Frho = Frho;
+        a.bvf = bvf; a.normal = normal; a.mass = mass; a.nu = nu; a.type = type; a.solid = solid; a.C = C; a.Q = Q;
+        a.Dmat = Dmat; a.Ndense = Ndense; a.rx = rx; a.bc = bc; a.deg = deg; a.nidx = nidx; a.nr = nr; a.err = derr;
+        return a;
+    }
+
+    RdmeArgs rdme_args(unsigned step) {
+        RdmeArgs a{};
+        a.n = n; a.S = S; a.R = R; a.T = T; a.W = W;
+        a.mass = mass; a.rho = rho; a.type = type; a.xx = xx; a.rrate = rrate; a.srrate = srrate; a.sdrate = sdrate;
+        a.Ddiag = Ddiag; a.Dmat = Dmat; a.rx = rx; a.jcN = jcN; a.irN = irN; a.prN = prN; a.jcG = jcG; a.irG = irG;
+        a.deg = deg; a.nidx = nidx; a.Dij = Dij;
+        a.qtime = qtime; a.tmin = tmin; a.targ = targ; a.nlev = nlev;
+        for (int l = 0; l < 8; l++) { a.lev_off[l] = lev_off[l]; a.lev_n[l] = lev_n[l]; }
+        a.qs = qs;
+        a.rng_kind = rng_kind; a.seed = seed; a.stream_id = 0; a.words = d_words; a.expo = d_expo; a.n_words = n_words;
+        a.textbook = (flags & SSASDPD_FLAG_TEXTBOOK_SELECTION) ? 1 : 0;
+        a.trace = trace_cap > 0 ? d_trace : nullptr;
+        a.trace_cap = trace_cap;
+        a.max_events = 1ULL << 26;   // per step launch; beyond it the run stops with CAPACITY
+        a.t0 = step * dt;
+        a.dt = dt;
+        a.endTime = static_domain ? nt * dt : a.t0 + dt;
+        a.step = step;
+        a.psum_out = d_psum;
+        return a;
+    }
+
+    int check(unsigned step) {
+        DevErr e = read_err();
+        if (e.code) {
+            stats.bad_particle = e.particle == INT32_MAX ? -1 : e.particle;
+            char buf[160];
+            snprintf(buf, sizeof buf, "step %u: particle %d", step, stats.bad_particle);
+            return set_error(e.code, e.code == SSASDPD_ERR_NAN_STATE ? std::string("nan/inf state at ") + buf
+                                                                        : std::string("NaN D_ij at ") + buf);
+        }
+        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
+            QueueState q;
+            d2h(st, &q, qs, 1);
+            HIP_OK(hipStreamSynchronize(st));
+            stats.reactions = q.reactions;
+            stats.diffusions = q.diffusions;
+            stats.rng_words_used = q.rng_pos;
+            if (q.err) {
+                char buf[96];
+                snprintf(buf, sizeof buf, "SSA failed at step %u", step);
+                return set_error(q.err, buf);
+            }
+        }
+        return 0;
+    }
+
+    int do_step() {
+        const unsigned step = cur;
+        const bool moving = static_domain == 0;
+        auto t_start = std::chrono::steady_clock::now();
+        if (step == 0 || moving) {
+            bin_tree_points();
+            search();                              // substep 0 lists (simulate.cpp:61-63)
+        }
+        SdpdArgs sa = sdpd_args(step);
+        sdpd_predict(st, sa);                      // substep 0
+        if (step > 0 && moving) {
+            search();                              // substep 1 lists (simulate.cpp:121-123)
+            sa = sdpd_args(step);
+        }
+        sdpd_forces(st, sa);                       // substep 1
+        sdpd_correct(st, sa);                      // substep 2
+        sdpd_bvf(st, sa);
+        HIP_OK(hipStreamSynchronize(st));
+        auto t_sdpd = std::chrono::steady_clock::now();
+        int rc = 0;
+        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
+            RdmeArgs ra = rdme_args(step);
+            if (moving || !rdme_initialized) {
+                rdme_rates(st, ra);
+                rdme_queue_build(st, ra, scan_tmp, qwork);
+                QueueState q;
+                d2h(st, &q, qs, 1);
+                HIP_OK(hipStreamSynchronize(st));
+                if (q.initialized == 1) rdme_initialized = true;
+            }
+            auto t_rates = std::chrono::steady_clock::now();
+            stats.ms_rates += std::chrono::duration<double, std::milli>(t_rates - t_sdpd).count();
+            rdme_take_step(st, ra);
+            HIP_OK(hipStreamSynchronize(st));
+            stats.ms_ssa += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rates).count();
+        }
+        rc = check(step);
+        stats.ms_sdpd += std::chrono::duration<double, std::milli>(t_sdpd - t_start).count();
+        cur++;
+        stats.steps = cur;
+        stats.particle_steps = (uint64_t)cur * n;
+        stats.max_degree = max_deg_seen;
+        return rc;
+    }
+
+    void snapshot(ssasdpd_outputs *o, int k) {
+        size_t nn = n;
+        std::vector<double> tmp;
+        if (o->xx && S > 0) d2h(st, o->xx + (size_t)k * nn * S, xx, nn * S);
+        if (o->C && Sc > 0) d2h(st, o->C + (size_t)k * nn * Sc, C, nn * Sc);
+        if (o->rho) d2h(st, o->rho + (size_t)k * nn, rho, nn);
+        if (o->bvf_phi) d2h(st, o->bvf_phi + (size_t)k * nn, bvf, nn);
+        if (o->nu) d2h(st, o->nu + (size_t)k * nn, nu, nn);
+        if (o->x) {
+            tmp.resize(3 * nn);
+            d2h(st, tmp.data(), x, 3 * nn);
+            HIP_OK(hipStreamSynchronize(st));
+            from_soa3(tmp, o->x + (size_t)k * nn * 3, n);
+        }
+        if (o->v) {
+            tmp.resize(3 * nn);
+            d2h(st, tmp.data(), v, 3 * nn);
+            HIP_OK(hipStreamSynchronize(st));
+            from_soa3(tmp, o->v + (size_t)k * nn * 3, n);
+        }
+        if (o->steps) o->steps[k] = cur;
+        HIP_OK(hipStreamSynchronize(st));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+#define GUARD_BEGIN try {
+#define GUARD_END                                                            \
+    }                                                                        \
+    catch (const HipError &e) {                                              \
+        return set_error(SSASDPD_ERR_HIP, e.msg);                            \
+    }                                                                        \
+    catch (const std::exception &e) {                                        \
+        return set_error(SSASDPD_ERR_HIP, e.what());                         \
+    }
+
+static int validate(const ssasdpd_desc *d) {
+    if (!d) return set_error(SSASDPD_ERR_BAD_DESC, "null descriptor");
+    if (d->abi_version != SSASDPD_ABI_VERSION) return set_error(SSASDPD_ERR_BAD_DESC, "ABI version mismatch");
+    if (d->n <= 0) return set_error(SSASDPD_ERR_BAD_DESC, "n must be positive");
+    if (d->dimension < 1 || d->dimension > 3) return set_error(SSASDPD_ERR_BAD_DESC, "dimension must be 1, 2 or 3");
+    if (d->n_types < 1 || d->n_types > 63) return set_error(SSASDPD_ERR_BAD_DESC, "n_types must be in 1..63");
+    if (!d->x || !d->type || !d->mass || !d->rho || !d->nu || !d->fixed)
+        return set_error(SSASDPD_ERR_BAD_DESC, "missing particle arrays");
+    if (!(d->h > 0)) return set_error(SSASDPD_ERR_BAD_DESC, "h (basis function width) must be positive");
+    if (d->n_species > 16 || d->n_reactions > 16 || d->n_chem_species > 16)
+        return set_error(SSASDPD_ERR_UNSUPPORTED, "at most 16 species and 16 reactions");
+    if (d->n_species > 0 && (!d->u0 || !d->diffusion_matrix || !d->jcG))
+        return set_error(SSASDPD_ERR_BAD_DESC, "missing chemistry arrays");
+    if (d->n_reactions > 0 && (!d->reactions || !d->jcN))
+        return set_error(SSASDPD_ERR_BAD_DESC, "missing reaction arrays");
+    for (int i = 0; i < d->n; i++)
+        if (d->type[i] < 1 || d->type[i] > d->n_types)
+            return set_error(SSASDPD_ERR_BAD_DESC, "particle type out of range 1..n_types");
+    int Rmax = std::max(d->n_reactions, d->n_chem_reactions);
+    int Smax = std::max(d->n_species, d->n_chem_species);
+    for (int r = 0; r < Rmax; r++) {
+        const ssasdpd_reaction &q = d->reactions[r];
+        if (q.kind < 0 || q.kind > 3 || q.a < 0 || q.a >= std::max(Smax, 1) || q.b < 0 || q.b >= std::max(Smax, 1))
+            return set_error(SSASDPD_ERR_BAD_DESC, "bad reaction table entry");
+    }
+    return 0;
+}
+
+extern "C" {
+
+const char *ssasdpd_last_error(void) { return g_last_error.c_str(); }
+int ssasdpd_abi_version(void) { return SSASDPD_ABI_VERSION; }
+
+int ssasdpd_create(const ssasdpd_desc *d, ssasdpd_engine **out) {
+    if (!out) return set_error(SSASDPD_ERR_BAD_DESC, "null out");
+    *out = nullptr;
+    int rc = validate(d);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(SSASDPD_ERR_HIP, "no HIP device");
+    ssasdpd_engine *e = new ssasdpd_engine();
+    try {
+        e->setup(d);
+        e->reset(nullptr);
+    } catch (const HipError &err) {
+        delete e;
+        return set_error(SSASDPD_ERR_HIP, err.msg);
+    }
+    *out = e;
+    return 0;
+}
+
+void ssasdpd_destroy(ssasdpd_engine *e) { delete e; }
+
+int ssasdpd_num_snapshots(const ssasdpd_desc *d, int32_t *n_snap) {
+    uint32_t next = 0;
+    int idx = 0, k = 0;
+    for (uint32_t step = 0; step < d->nt; step++) {
+        if (step >= next) {
+            k++;
+            next = idx < d->n_output_steps ? d->output_steps[idx] : 0xFFFFFFFFu;
+            idx++;
+        }
+    }
+    *n_snap = k + 1;
+    return 0;
+}
+
+int ssasdpd_reset(ssasdpd_engine *e, const ssasdpd_rng *r) {
+    GUARD_BEGIN
+    e->reset(r);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps) {
+    GUARD_BEGIN
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t k = 0; k < n_steps && e->cur < e->nt; k++) {
+        int rc = e->do_step();
+        if (rc) return rc;
+    }
+    e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
+    GUARD_BEGIN
+    e->reset(r);
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t next = 0;
+    size_t idx = 0;
+    int k = 0;
+    for (uint32_t step = 0; step < e->nt; step++) {
+        if (step >= next) {
+            if (o && k < o->n_snapshots) e->snapshot(o, k);
+            k++;
+            next = idx < e->out_steps.size() ? e->out_steps[idx] : 0xFFFFFFFFu;
+            idx++;
+        }
+        int rc = e->do_step();
+        if (rc) return rc;
+    }
+    if (o && k < o->n_snapshots) e->snapshot(o, k);
+    e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, uint32_t *counts, int32_t n_snap) {
+    (void)e; (void)n_traj; (void)seeds; (void)counts; (void)n_snap;
+    return set_error(SSASDPD_ERR_UNSUPPORTED, "batched runs are not available in this build");
+}
+
+int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s) {
+    *s = e->stats;
+    return 0;
+}
+
+int ssasdpd_get_state(ssasdpd_engine *e, ssasdpd_state *s) {
+    GUARD_BEGIN
+    const int n = e->n;
+    hipStream_t st = e->st;
+    std::vector<double> tmp(3 * (size_t)n);
+    auto v3 = [&](double *dst, double *src) {
+        if (!dst) return;
+        d2h(st, tmp.data(), src, 3 * (size_t)n);
+        HIP_OK(hipStreamSynchronize(st));
+        from_soa3(tmp, dst, n);
+    };
+    v3(s->x, e->x); v3(s->v, e->v); v3(s->vt, e->vt); v3(s->F, e->F); v3(s->Fbp, e->Fbp);
+    auto v1 = [&](double *dst, double *src, size_t cnt) { if (dst && src) d2h(st, dst, src, cnt); };
+    v1(s->rho, e->rho, n); v1(s->Frho, e->Frho, n); v1(s->bvf_phi, e->bvf, n); v1(s->nu, e->nu, n);
+    v1(s->old_rho, e->old_rho, n);
+    if (e->Sc) { v1(s->C, e->C, (size_t)n * e->Sc); v1(s->Q, e->Q, (size_t)n * e->Sc); }
+    if (e->S) {
+        if (s->xx) d2h(st, s->xx, e->xx, (size_t)n * e->S);
+        v1(s->srrate, e->srrate, n); v1(s->sdrate, e->sdrate, n);
+        if (e->R) v1(s->rrate, e->rrate, (size_t)n * e->R);
+        v1(s->Ddiag, e->Ddiag, (size_t)n * e->S);
+        v1(s->next_time, e->qtime, n);
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx, double *dist, double *dWdr, double *Dij) {
+    GUARD_BEGIN
+    const int n = e->n, W = e->W;
+    std::vector<int> deg(n);
+    d2h(e->st, deg.data(), e->deg, n);
+    HIP_OK(hipStreamSynchronize(e->st));
+    int64_t off = 0;
+    for (int i = 0; i < n; i++) { row_ptr[i] = off; off += deg[i]; }
+    row_ptr[n] = off;
+    if (!idx) return 0;
+    std::vector<int> ni((size_t)W * n);
+    std::vector<double> nr((size_t)W * n), nd((size_t)W * n);
+    d2h(e->st, ni.data(), e->nidx, (size_t)W * n);
+    d2h(e->st, nr.data(), e->nr, (size_t)W * n);
+    d2h(e->st, nd.data(), e->Dij, (size_t)W * n);
+    HIP_OK(hipStreamSynchronize(e->st));
+    const double h = e->h, a = e->alpha;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < deg[i]; k++) {
+            size_t s = (size_t)k * n + i;
+            int64_t o = row_ptr[i] + k;
+            idx[o] = ni[s];
+            double r = nr[s];
+            if (dist) dist[o] = r;
+            if (dWdr) {
+                double R = r / h;
+                dWdr[o] = a * (-12 * r / (h * h)) * ((1 - R) * (1 - R));   // particle.cpp:178
+            }
+            if (Dij) Dij[o] = nd[s];
+        }
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m, int32_t dim, double h,
+                      int64_t *row_ptr, int32_t *idx, double *r2) {
+    GUARD_BEGIN
+    if (n <= 0 || m < 0 || dim < 1 || dim > 3 || !(h > 0)) return set_error(SSASDPD_ERR_BAD_DESC, "bad fr_search args");
+    hipStream_t st;
+    HIP_OK(hipStreamCreate(&st));
+    std::vector<double> hp = to_soa3(pts, n), hq = to_soa3(q, m);
+    double *dp = dalloc<double>(3 * (size_t)n), *dq = dalloc<double>(3 * (size_t)std::max(m, 1));
+    double *sp = dalloc<double>(3 * (size_t)n), *bb = dalloc<double>(8);
+    unsigned long long *bbt = dalloc<unsigned long long>(8);
+    int *srt = dalloc<int>(n), *cof = dalloc<int>(n);
+    h2d(st, dp, hp.data(), 3 * (size_t)n);
+    h2d(st, dq, hq.data(), 3 * (size_t)m);
+    compute_bbox(st, dp, n, dim, bb, bbt);
+    double hb[6];
+    d2h(st, hb, bb, 6);
+    HIP_OK(hipStreamSynchronize(st));
+    Grid g;
+    grid_setup(hb, n, dim, h, g);
+    int *cc = dalloc<int>(g.ncell + 1), *cs = dalloc<int>(g.ncell + 2), *cf = dalloc<int>(g.ncell + 1);
+    void *stmp;
+    HIP_OK(hipMalloc(&stmp, std::max(scan_tmp_bytes(g.ncell + 8), scan_tmp_bytes(n + 8))));
+    bin_points(st, dp, n, dim, g, cof, cc, cs, cf, srt, sp, stmp);
+    KdDev kd;
+    kd.parent = dalloc<int>(2 * (size_t)n + 2); kd.depth = dalloc<int>(2 * (size_t)n + 2);
+    kd.cd = dalloc<int>(2 * (size_t)n + 2); kd.cv = dalloc<double>(2 * (size_t)n + 2);
+    kd.is_hi = dalloc<int>(2 * (size_t)n + 2); kd.leaf_of = dalloc<int>(n);
+    size_t kws = kd_work_bytes(n);
+    void *kw;
+    HIP_OK(hipMalloc(&kw, kws));
+    kd_build(st, dp, n, dim, kd, kw, kws);
+    DevErr *de = dalloc<DevErr>(1);
+    int W = 16;
+    int *oi = nullptr, *dg = dalloc<int>(std::max(m, 1));
+    double *orr = nullptr;
+    for (int attempt = 0; attempt < 6; attempt++) {
+        dfree(oi); dfree(orr);
+        oi = dalloc<int>((size_t)W * std::max(m, 1));
+        orr = dalloc<double>((size_t)W * std::max(m, 1));
+        DevErr e0{0, INT32_MAX, 0, 0, 0};
+        h2d(st, de, &e0, 1);
+        if (m > 0) fr_search(st, dq, m, sp, srt, cs, n, dim, h, g, &kd, W, oi, orr, dg, de, nullptr);
+        DevErr e;
+        d2h(st, &e, de, 1);
+        HIP_OK(hipStreamSynchronize(st));
+        if (!e.overflow) break;
+        W = ((e.max_deg + 7) / 8) * 8 + 8;
+    }
+    std::vector<int> hd(std::max(m, 1));
+    d2h(st, hd.data(), dg, m);
+    HIP_OK(hipStreamSynchronize(st));
+    int64_t off = 0;
+    for (int i = 0; i < m; i++) { row_ptr[i] = off; off += hd[i]; }
+    row_ptr[m] = off;
+    if (idx) {
+        std::vector<int> hi((size_t)W * m);
+        std::vector<double> hr((size_t)W * m);
+        d2h(st, hi.data(), oi, (size_t)W * m);
+        d2h(st, hr.data(), orr, (size_t)W * m);
+        HIP_OK(hipStreamSynchronize(st));
+        for (int i = 0; i < m; i++)
+            for (int k = 0; k < hd[i]; k++) {
+                idx[row_ptr[i] + k] = hi[(size_t)k * m + i];
+                r2[row_ptr[i] + k] = hr[(size_t)k * m + i];
+            }
+    }
+    dfree(dp); dfree(dq); dfree(sp); dfree(bb); dfree(bbt); dfree(srt); dfree(cof); dfree(cc); dfree(cs); dfree(cf);
+    (void)hipFree(stmp); (void)hipFree(kw);
+    dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
+    dfree(de); dfree(oi); dfree(orr); dfree(dg);
+    (void)hipStreamDestroy(st);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_set_trace_capacity(ssasdpd_engine *e, uint64_t cap) {
+    GUARD_BEGIN
+    dfree(e->d_trace);
+    e->trace_cap = cap;
+    if (cap) e->d_trace = dalloc<ssasdpd_event>(cap);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_get_trace(ssasdpd_engine *e, ssasdpd_event *ev, uint64_t cap, uint64_t *n_events) {
+    GUARD_BEGIN
+    QueueState q{};
+    if (e->qs) {
+        d2h(e->st, &q, e->qs, 1);
+        HIP_OK(hipStreamSynchronize(e->st));
+    }
+    *n_events = q.events_traced;
+    uint64_t m = std::min<uint64_t>(std::min<uint64_t>(cap, e->trace_cap), q.events_traced);
+    if (ev && m) {
+        d2h(e->st, ev, e->d_trace, m);
+        HIP_OK(hipStreamSynchronize(e->st));
+    }
+    return 0;
+    GUARD_END
+}
+
+// std::mt19937_64 (Matsumoto & Nishimura 2004) and libstdc++'s
+// exponential_distribution of each word, on the host.
+int ssasdpd_mt19937_64_stream(uint64_t seed, uint64_t n, uint64_t *words, double *expo) {
+    uint64_t mt[312];
+    mt[0] = seed;
+    for (int i = 1; i < 312; i++) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+    int mti = 312;
+    for (uint64_t k = 0; k < n; k++) {
+        if (mti >= 312) {
+            for (int i = 0; i < 312; i++) {
+                uint64_t y = (mt[i] & 0xFFFFFFFF80000000ULL) | (mt[(i + 1) % 312] & 0x7FFFFFFFULL);
+                uint64_t z = mt[(i + 156) % 312] ^ (y >> 1);
+                if (y & 1ULL) z ^= 0xB5026F5AA96619E9ULL;
+                mt[i] = z;
+            }
+            mti = 0;
+        }
+        uint64_t y = mt[mti++];
+        y ^= (y >> 29) & 0x5555555555555555ULL;
+        y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+        y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+        y ^= (y >> 43);
+        if (words) words[k] = y;
+        if (expo) {
+            double c = (double)y * 1.0 / 18446744073709551616.0;
+            if (c >= 1.0) c = std::nextafter(1.0, 0.0);
+            expo[k] = -std::log(1.0 - c) / 1.0;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+// Internal declarations shared by the engine's translation units.
+// Device memory layout (all arrays live in HBM for the whole run):
+//   particles: component-major FP64 [3][n] for x, v, vt, F, Fbp, normal;
+//              FP64 [n] rho, old_rho, Frho, mass, nu, bvf; int32 type, solid
+//   deterministic species: FP64 C[n][Sc], Q[n][Sc]
+//   neighbour lists: ELL, column-major [W][n] for idx (int32), r (FP64 |x_i-x_j|)
+//              and Dij (FP64); deg[n]
+//   RDME: xx uint32 [n][S], rrate FP64 [n][R], srrate/sdrate FP64 [n],
+//         Ddiag FP64 [n][S], firing times FP64 [n] + a 64-ary min tree
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ssasdpd.h"
+
+namespace ssa {
+
+#define HIP_OK(expr)                                                              \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) throw ::ssa::HipError(#expr, _e, __FILE__, __LINE__); \
+    } while (0)
+
+struct HipError {
+    std::string msg;
+    HipError(const char *what, hipError_t e, const char *file, int line) {
+        msg = std::string(what) + ": " + hipGetErrorString(e) + " at " + file + ":" + std::to_string(line);
+    }
+};
+
+constexpr int WAVE = 64;
+constexpr int TREE_FAN = 64;
+
+// Device-side error record.
+struct DevErr {
+    int code;          // ssasdpd_status
+    int particle;      // offending particle (NaN), or -1
+    unsigned step;
+    int max_deg;       // largest neighbour count seen by the last search
+    int overflow;      // a list did not fit in the ELL width
+};
+
+// Mass-action reaction (same as ssasdpd_reaction, device copy).
+struct Rxn {
+    int kind, a, b, pad;
+    double k;
+    uint64_t mask;
+};
+
+struct BcRule {
+    double lo[3], hi[3];
+    int type, target, species, pad;
+    double value[3];
+};
+
+// Per-trajectory SSA queue/time-window state (restates the scalar state of
+// NRMConstant_v5, ENG/include/NRMConstant_v5.hpp:58-107).
+struct QueueState {
+    double L, U, endTime;
+    unsigned long long nbins, rxnCount, active;
+    unsigned long long rng_pos;      // replay: next word; seeded: draw counter
+    unsigned long long reactions, diffusions, events_traced;
+    int initialized;
+    int err;
+    int pad[2];
+};
+
+// Kd-tree of the tree points (ANN sliding-midpoint, bucket 1), flattened.
+struct KdDev {
+    int *parent = nullptr;   // [2n]
+    int *depth = nullptr;    // [2n]
+    int *cd = nullptr;       // [2n]
+    double *cv = nullptr;    // [2n]
+    int *is_hi = nullptr;    // [2n]
+    int *leaf_of = nullptr;  // [n] leaf node of each point
+    int n_nodes = 0;
+    bool valid = false;
+};
+
+// Cell grid over the tree points.
+struct Grid {
+    double origin[3];
+    double inv_cell;
+    int dims[3];
+    long long ncell;
+};
+
+struct Bufs;   // defined in engine.hip
+
+// ---- launch wrappers (implemented in nbr.hip / sdpd.hip / ssa.hip) -------
+// Neighbour search: bins tree points pts[3][n] into cells, then for each
+// query q[3][m] collects accepted points, sorts them in ANN order and writes
+// ELL rows (idx, r2) of width W.  Returns via err->overflow when W is too
+// small.
+void grid_setup(const double *pts_host_bbox, int n, int dim, double h, Grid &g);
+void bin_points(hipStream_t s, const double *pts, int n, int dim, const Grid &g, int *cell_of,
+                int *cell_count, int *cell_start, int *cell_fill, int *sorted, double *spts,
+                void *scan_tmp);
+void compute_bbox(hipStream_t s, const double *pts, int n, int dim, double *d_bbox /*[6]*/,
+                  unsigned long long *tmp);
+void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted,
+               const int *cell_start, int n, int dim, double h, const Grid &g, const KdDev *kd,
+               int W, int *out_idx, double *out_r2, int *deg, DevErr *err, const int *query_order);
+void kd_build(hipStream_t s, const double *pts, int n, int dim, KdDev &kd, void *work, size_t work_bytes);
+size_t kd_work_bytes(int n);
+size_t scan_tmp_bytes(long long n);
+void exclusive_scan_i32(hipStream_t s, const int *in, int *out, long long n, void *tmp);
+
+// D_ij and r from r2 (add_to_neighbor_list, ENG/src/particle.cpp:150-210).
+void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2,
+                  double *r, double *Dij, const double *mass, const double *rho, double h,
+                  DevErr *err);
+
+// SDPD substeps.
+struct SdpdArgs {
+    int n, dim, Sc, Rc, T, static_domain, nbc, W;
+    unsigned step;
+    double dt, h, rho0, c0, P0, alpha;
+    double g[3];
+    double *x, *v, *vt, *F, *Fbp, *rho, *old_rho, *Frho, *bvf, *normal;
+    const double *mass;
+    double *nu;
+    const int *type, *solid;
+    double *C, *Q;
+    const double *Dmat;      // [S][T] species-major
+    const int *Ndense;       // [Sc][Rc]
+    const Rxn *rx;
+    const BcRule *bc;
+    const int *deg, *nidx;
+    const double *nr;
+    DevErr *err;
+};
+void sdpd_predict(hipStream_t s, const SdpdArgs &a);
+void sdpd_forces(hipStream_t s, const SdpdArgs &a);
+void sdpd_correct(hipStream_t s, const SdpdArgs &a);
+void sdpd_bvf(hipStream_t s, const SdpdArgs &a);
+
+// RDME.
+struct RdmeArgs {
+    int n, S, R, T, W;
+    const double *mass, *rho;
+    const int *type;
+    uint32_t *xx;
+    double *rrate, *srrate, *sdrate, *Ddiag;
+    const double *Dmat;
+    const Rxn *rx;
+    const int *jcN, *irN, *prN, *jcG, *irG;
+    const int *deg, *nidx;
+    const double *Dij;
+    // queue
+    double *qtime;           // level 0
+    double *tmin;            // upper levels, concatenated
+    int *targ;
+    int nlev;                // number of upper levels
+    int lev_off[8];          // offset of each upper level in tmin/targ
+    int lev_n[8];            // nodes per level (level 0 = n)
+    QueueState *qs;
+    // random stream
+    int rng_kind;            // 0 seeded (Philox), 1 replay
+    unsigned long long seed;
+    unsigned long long stream_id;
+    const uint64_t *words;
+    const double *expo;
+    unsigned long long n_words;
+    // options
+    int textbook;
+    ssasdpd_event *trace;
+    unsigned long long trace_cap;
+    unsigned long long max_events;
+    double t0, dt, endTime;
+    unsigned step;
+    double *psum_out;
+    int *active_out;
+};
+void rdme_rates(hipStream_t s, const RdmeArgs &a);
+void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
+void rdme_take_step(hipStream_t s, const RdmeArgs &a);
+
+}  // namespace ssa

@@ -1,0 +1,168 @@
+"""Python handle of the GPU engine (libssasdpd.so through the C ABI)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .lowering import DescHandle, output_schedule
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def make_rng(seed=None, words=None, expo=None):
+    """Random stream descriptor: seeded (GPU generator) or replay."""
+    r = _abi.Rng()
+    if words is None:
+        r.kind = 0
+        r.seed = int(seed if seed is not None else 0) & 0xFFFFFFFFFFFFFFFF
+    else:
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        expo = np.ascontiguousarray(expo, dtype=np.float64)
+        r.kind = 1
+        r.words = _p(words, C.c_uint64)
+        r.expo = _p(expo, C.c_double)
+        r.n_words = words.size
+    return r, (words, expo)
+
+
+def mt19937_64_stream(seed, n):
+    """std::mt19937_64(seed) words and their exponential variates (host libm)."""
+    lib = _abi.load_library()
+    w = np.zeros(n, np.uint64)
+    e = np.zeros(n, np.float64)
+    lib.ssasdpd_mt19937_64_stream(int(seed) & 0xFFFFFFFFFFFFFFFF, n, _p(w, C.c_uint64), _p(e, C.c_double))
+    return w, e
+
+
+class Engine:
+    """One model on the GPU; mirrors the engine life cycle of the C ABI."""
+
+    def __init__(self, sa, trace_capacity=0):
+        self.lib = _abi.load_library()
+        self.dh = DescHandle(sa)
+        self.sa = sa
+        h = C.c_void_p()
+        _abi.check(self.lib, self.lib.ssasdpd_create(self.dh.byref(), C.byref(h)), "ssasdpd_create")
+        self.h = h
+        if trace_capacity:
+            self.set_trace(trace_capacity)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ssasdpd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def S(self):
+        return self.dh.desc.n_species
+
+    @property
+    def Sc(self):
+        return self.dh.desc.n_chem_species
+
+    @property
+    def R(self):
+        return self.dh.desc.n_reactions
+
+    def reset(self, seed=None, words=None, expo=None):
+        r, self._keep = make_rng(seed, words, expo)
+        _abi.check(self.lib, self.lib.ssasdpd_reset(self.h, C.byref(r)), "ssasdpd_reset")
+
+    def advance(self, n):
+        _abi.check(self.lib, self.lib.ssasdpd_advance(self.h, n), "ssasdpd_advance")
+
+    def set_trace(self, cap):
+        _abi.check(self.lib, self.lib.ssasdpd_set_trace_capacity(self.h, cap), "trace")
+
+    def trace(self, cap):
+        ev = np.zeros(cap, _abi.EVENT_DTYPE)
+        n = C.c_uint64()
+        _abi.check(self.lib, self.lib.ssasdpd_get_trace(self.h, _p(ev, _abi.Event), cap, C.byref(n)), "trace")
+        return ev[: min(cap, n.value)], n.value
+
+    def stats(self):
+        s = _abi.Stats()
+        self.lib.ssasdpd_stats_get(self.h, C.byref(s))
+        return s
+
+    def state(self):
+        n = self.sa.n
+        S, R, Sc = self.S, self.R, self.Sc
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "vt", "F", "Fbp")}
+        out.update({k: np.zeros(n) for k in ("rho", "Frho", "bvf_phi", "nu", "old_rho", "srrate", "sdrate",
+                                              "next_time")})
+        out["C"] = np.zeros((n, Sc))
+        out["Q"] = np.zeros((n, Sc))
+        out["xx"] = np.zeros((n, S), np.uint32)
+        out["rrate"] = np.zeros((n, R))
+        out["Ddiag"] = np.zeros((n, S))
+        st = _abi.State()
+        for k, a in out.items():
+            setattr(st, k, _p(a, C.c_uint32 if a.dtype == np.uint32 else C.c_double))
+        _abi.check(self.lib, self.lib.ssasdpd_get_state(self.h, C.byref(st)), "get_state")
+        return out
+
+    def neighbors(self):
+        n = self.sa.n
+        rp = np.zeros(n + 1, np.int64)
+        _abi.check(self.lib, self.lib.ssasdpd_get_neighbors(self.h, _p(rp, C.c_int64), None, None, None, None), "nbrs")
+        m = int(rp[-1])
+        idx = np.zeros(m, np.int32)
+        dist, dwdr, dij = np.zeros(m), np.zeros(m), np.zeros(m)
+        _abi.check(self.lib, self.lib.ssasdpd_get_neighbors(
+            self.h, _p(rp, C.c_int64), _p(idx, C.c_int32), _p(dist, C.c_double), _p(dwdr, C.c_double),
+            _p(dij, C.c_double)), "nbrs")
+        return rp, idx, dist, dwdr, dij
+
+    def run(self, seed=None, words=None, expo=None, fields=("xx", "C", "x", "v", "rho")):
+        """Full trajectory with the reference's snapshot schedule."""
+        sa = self.sa
+        steps = output_schedule(sa.output_steps, sa.nt)
+        k = steps.size
+        n = sa.n
+        res = {}
+        o = _abi.Outputs()
+        o.n_snapshots = k
+        if "xx" in fields:
+            res["xx"] = np.zeros((k, n, self.S), np.uint32)
+            o.xx = _p(res["xx"], C.c_uint32)
+        if "C" in fields:
+            res["C"] = np.zeros((k, n, self.Sc))
+            o.C = _p(res["C"], C.c_double)
+        for f in ("x", "v"):
+            if f in fields:
+                res[f] = np.zeros((k, n, 3))
+                setattr(o, f, _p(res[f], C.c_double))
+        for f, attr in (("rho", "rho"), ("bvf_phi", "bvf_phi"), ("nu", "nu")):
+            if f in fields:
+                res[f] = np.zeros((k, n))
+                setattr(o, attr, _p(res[f], C.c_double))
+        res["steps"] = np.zeros(k, np.uint32)
+        o.steps = _p(res["steps"], C.c_uint32)
+        r, keep = make_rng(seed, words, expo)
+        _abi.check(self.lib, self.lib.ssasdpd_run(self.h, C.byref(r), C.byref(o)), "ssasdpd_run")
+        return res
+
+
+def fr_search(pts, q, dim, h):
+    """Stand-alone GPU fixed-radius search (lists in ANN order)."""
+    lib = _abi.load_library()
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    m = q.shape[0]
+    rp = np.zeros(m + 1, np.int64)
+    _abi.check(lib, lib.ssasdpd_fr_search(_p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), m, dim, h,
+                                          _p(rp, C.c_int64), None, None), "fr_search")
+    idx = np.zeros(rp[-1], np.int32)
+    r2 = np.zeros(rp[-1], np.float64)
+    _abi.check(lib, lib.ssasdpd_fr_search(_p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), m, dim, h,
+                                          _p(rp, C.c_int64), _p(idx, C.c_int32), _p(r2, C.c_double)), "fr_search")
+    return rp, idx, r2
