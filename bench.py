@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark: the 1M-particle 3D SDPD + sSSA cylinder run on one MI355X.
+
+One step = neighbour search (cell binning), the three SDPD substeps, the
+diffusion jump-rate assembly and that step's exact SSA, i.e. one iteration
+of the reference's step loop (spatialpy/solvers/c_base/
+ssa_sdpd-c-simulation-engine/src/simulate_threads.cpp:232-281).
+
+Prints ONE JSON line (metric particle-steps/s, higher is better), with the
+SSA event rate, the roofline of the dominant HBM-bound kernels and the
+reference CPU engine timed on this host's cores beside it.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_particle_step(dim, S, E):
+    """Compulsory HBM traffic per particle-step of the SDPD + rate path:
+    read x, v, vt (3*8d), rho, mass, nu (24), type (4); write x, v, vt (3*8d),
+    rho (8); per species int32 count read + FP64 Ddiag write (12 S); per
+    neighbour int32 index + FP64 D_ij written once per step (12 E)."""
+    return (3 * 8 * dim + 8 + 8 + 8 + 4) + (3 * 8 * dim + 8) + 12 * S + 12 * E
+
+
+def build_workload(n_target, nt):
+    from spatialpy_amd import workloads
+    return workloads.cylinder_3d(n_target=n_target, nt=nt)
+
+
+def cpu_baseline(sa, steps, seconds_budget=30.0):
+    """Time the reference engine itself (oracle/_ref, compiled from the
+    reference sources in this repository's build step) on this host's cores,
+    on a bounded sample: the same workload for `steps` steps."""
+    threads = min(os.cpu_count() or 1, 16)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_O2")
+    from spatialpy_amd import lowering
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as d:
+            lowering.write_harness_dir(sa, d, seed=1, threads=threads, run_steps=steps)
+            r = subprocess.run([exe, d], capture_output=True, text=True, timeout=600)
+            line = [l for l in r.stdout.splitlines() if l.startswith("REF_TIMING")]
+            if r.returncode == 0 and line:
+                kv = dict(tok.split("=") for tok in line[0].split()[1:])
+                sec = float(kv["seconds"])
+                return {"value": sa.n * steps / sec, "unit": "particle-steps/s", "cores": threads,
+                        "kind": "reference",
+                        "sample": f"reference C++ engine (-O2, {threads} threads, run_simulation worker "
+                                  f"structure without VTK output) on the same {sa.n}-particle cylinder, "
+                                  f"{steps} steps from t=0: {sec:.2f} s",
+                        "ssa_events_per_s": (int(kv["reactions"]) + int(kv["diffusions"])) / sec}
+    # the repository's FP64 restatement (single thread) when the reference
+    # build is not present
+    from oracle import oracle_py
+    o = oracle_py.Oracle(sa, gauss_seidel=True)
+    o.reset(seed=1)
+    t = time.perf_counter()
+    o.advance(steps)
+    sec = time.perf_counter() - t
+    return {"value": sa.n * steps / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle restatement, 1 thread, {sa.n} particles, {steps} steps: {sec:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000, help="particles in the cylinder")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from spatialpy_amd.engine import Engine
+    from spatialpy_amd import _abi
+
+    total_steps = args.warmup + args.steps
+    sa = build_workload(args.n, total_steps + 1)
+    if args.ann_order:
+        sa.flags |= _abi.FLAG_ANN_ORDER_MOVING
+    eng = Engine(sa)
+    eng.reset(seed=12345)
+    eng.advance(args.warmup)
+    st0 = eng.stats()
+    ph0 = eng.phase_times()
+    t0 = time.perf_counter()
+    eng.advance(args.steps)
+    t1 = time.perf_counter()
+    st1 = eng.stats()
+    ph1 = eng.phase_times()
+    sec = t1 - t0
+    n = sa.n
+    value = n * args.steps / sec
+    events = (st1.reactions + st1.diffusions) - (st0.reactions + st0.diffusions)
+    dph = {k: (ph1[k][0] - ph0[k][0], ph1[k][1] - ph0[k][1]) for k in ph1}
+
+    # roofline of the HBM-bound particle path (search + SDPD + rates) and of
+    # the force kernel alone
+    E = 6
+    B = algorithmic_bytes_per_particle_step(3, sa.n_species, E)
+    hbm_ms = dph["search"][0] + dph["sdpd"][0] + dph["forces"][0] + dph["rates_queue"][0] + dph["bin_tree"][0]
+    achieved = B * n * args.steps / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
+    ssa_ms = dph["ssa"][0]
+    dominant = "ssa_event_loop" if ssa_ms > hbm_ms else "sdpd_particle_path"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernels": "binning + fixed-radius search + D_ij + SDPD substeps + rates/queue build",
+                "algorithmic_bytes_per_particle_step": B,
+                "ms_per_step": round(hbm_ms / args.steps, 4)}
+    out = {
+        "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D SDPD+sSSA cylinder r=1 L=4, {n} particles, h=1.3*spacing, c0=20, "
+                               f"nu=1e-2, inflow 1.0, A+B->C k=1e-2 with {n} A + {n} B, dt=2.5e-4, "
+                               f"plain SDPD (no surrogate), seeded GPU stream",
+                   "particles": n, "ann_order_moving": bool(args.ann_order)},
+        "ssa_events_per_s": events / sec,
+        "ssa_events": int(events),
+        "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in dph.items()},
+        "dominant": dominant,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        from spatialpy_amd import workloads
+        csa = build_workload(args.n, args.cpu_steps + 1)
+        out["cpu_baseline"] = cpu_baseline(csa, args.cpu_steps)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -209,6 +209,13 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 
+/* GPU time per phase since the last reset, from HIP events on the engine's
+ * stream: 0 binning + kd-tree, 1 fixed-radius search + D_ij, 2 SDPD
+ * substeps 0 and 2, 3 rates + event-queue build, 4 SSA event loop, 5 the
+ * pairwise-force kernel.  Returns the number of phases.  (No reference
+ * counterpart: the reference has only gprof hooks, solver.py:421-430.) */
+int ssasdpd_phase_times(ssasdpd_engine *e, double *ms, uint64_t *launches, int32_t n_phases);
+
 /* ---- state access (parity tests) ---------------------------------------- */
 typedef struct ssasdpd_state {
     double *x, *v, *vt;       /* [n][3] each (any may be NULL)                  */

@@ -34,6 +34,7 @@
 #include <map>
 #include <ctime>
 #include <unistd.h>
+#include <pthread.h>
 #include <random>
 #include <string>
 #include <vector>
@@ -48,7 +49,6 @@ std::mt19937_64 rng;
 
 namespace Spatialpy {
 void buildKDTree(ParticleSystem *system);
-void run_simulation(int num_threads, ParticleSystem *system);
 }
 
 namespace {
@@ -302,16 +302,57 @@ int main(int argc, char **argv) {
 
     const int threads = g_meta.count("threads") ? geti("threads") : 0;
     if (threads > 0) {
-        /* timing mode: the reference's own threaded driver, as the generated
-         * main() calls it (propensity_file_template.cpp:140); VTK files go
-         * to the current directory */
+        /* timing mode: the worker structure of run_simulation
+         * (simulate_threads.cpp:122-281) -- contiguous particle blocks per
+         * thread, barriers between substeps, kd-tree build and RDME on the
+         * master -- without the VTK output thread, so only the engine's
+         * compute is timed. */
+        const unsigned steps = run_steps;
+        pthread_barrier_t bar;
+        pthread_barrier_init(&bar, nullptr, threads + 1);
+        struct Ctx { ParticleSystem *sys; int first, count; pthread_barrier_t *bar; unsigned steps; };
+        std::vector<Ctx> ctx(threads);
+        std::vector<pthread_t> th(threads);
+        int per = n / threads, left = n, first = 0;
+        for (int t = 0; t < threads; t++) {
+            int cnt = (t == threads - 1) ? left : per;
+            ctx[t] = Ctx{system, first, cnt, &bar, steps};
+            first += cnt;
+            left -= cnt;
+            pthread_create(&th[t], nullptr, [](void *arg) -> void * {
+                Ctx *c = (Ctx *)arg;
+                for (unsigned step = 0; step < c->steps; step++) {
+                    pthread_barrier_wait(c->bar);
+                    for (unsigned sub = 0; sub < 3; sub++) {
+                        for (int i = 0; i < c->count; i++)
+                            take_step(&c->sys->particles[c->first + i], c->sys, step, sub);
+                        pthread_barrier_wait(c->bar);
+                    }
+                }
+                return nullptr;
+            }, &ctx[t]);
+        }
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
-        run_simulation(threads, system);
+        double kd_s = 0, rdme_s = 0;
+        for (system->current_step = 0; system->current_step < steps; system->current_step++) {
+            struct timespec a, b;
+            clock_gettime(CLOCK_MONOTONIC, &a);
+            buildKDTree(system);
+            clock_gettime(CLOCK_MONOTONIC, &b);
+            kd_s += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+            pthread_barrier_wait(&bar);
+            for (unsigned sub = 0; sub < 3; sub++) pthread_barrier_wait(&bar);
+            clock_gettime(CLOCK_MONOTONIC, &a);
+            simulate_rdme(system, system->current_step);
+            clock_gettime(CLOCK_MONOTONIC, &b);
+            rdme_s += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+        }
         clock_gettime(CLOCK_MONOTONIC, &t1);
+        for (int t = 0; t < threads; t++) pthread_join(th[t], nullptr);
         double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
-        printf("REF_TIMING seconds=%.6f steps=%u n=%d reactions=%ld diffusions=%ld threads=%d\n", sec, nt, n,
-               system->total_reactions, system->total_diffusion, threads);
+        printf("REF_TIMING seconds=%.6f steps=%u n=%d reactions=%ld diffusions=%ld threads=%d kd_s=%.6f rdme_s=%.6f\n",
+               sec, steps, n, system->total_reactions, system->total_diffusion, threads, kd_s, rdme_s);
         fflush(stdout);
         _exit(0);
     }

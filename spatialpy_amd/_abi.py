@@ -128,10 +128,11 @@ def _declare(lib):
     lib.ssasdpd_set_trace_capacity.argtypes = [P, C.c_uint64]
     lib.ssasdpd_get_trace.argtypes = [P, C.POINTER(Event), C.c_uint64, _u64p]
     lib.ssasdpd_mt19937_64_stream.argtypes = [C.c_uint64, C.c_uint64, _u64p, _dp]
+    lib.ssasdpd_phase_times.argtypes = [P, _dp, _u64p, C.c_int32]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
                  "ssasdpd_run", "ssasdpd_run_batch", "ssasdpd_stats_get", "ssasdpd_get_state",
                  "ssasdpd_get_neighbors", "ssasdpd_fr_search", "ssasdpd_set_trace_capacity",
-                 "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream"]:
+                 "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream", "ssasdpd_phase_times"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

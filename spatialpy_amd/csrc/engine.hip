@@ -130,6 +130,40 @@ struct ssasdpd_engine {
     bool rdme_initialized = false;
     ssasdpd_stats stats{};
     int max_deg_seen = 0;
+    // GPU time per phase, HIP events on the engine stream: 0 binning + tree,
+    // 1 search + D_ij, 2 SDPD substeps, 3 rates + queue build, 4 SSA loop,
+    // 5 force kernel alone
+    static constexpr int NPH = 6;
+    hipEvent_t ev_a[NPH][2] = {}, ev_b[NPH][2] = {};
+    int ev_n[NPH] = {0};
+    double ph_ms[NPH] = {0};
+    uint64_t ph_launch[NPH] = {0};
+
+    void ph_begin(int p) {
+        if (!ev_a[p][0]) return;
+        int k = ev_n[p] & 1;
+        HIP_OK(hipEventRecord(ev_a[p][k], st));
+    }
+    void ph_end(int p) {
+        if (!ev_a[p][0]) return;
+        int k = ev_n[p] & 1;
+        HIP_OK(hipEventRecord(ev_b[p][k], st));
+        ev_n[p]++;
+        pending[p][k] = true;
+    }
+    bool pending[NPH][2] = {};
+    void ph_collect() {
+        for (int p = 0; p < NPH; p++)
+            for (int k = 0; k < 2; k++)
+                if (pending[p][k]) {
+                    float ms = 0;
+                    HIP_OK(hipEventSynchronize(ev_b[p][k]));
+                    HIP_OK(hipEventElapsedTime(&ms, ev_a[p][k], ev_b[p][k]));
+                    ph_ms[p] += ms;
+                    ph_launch[p]++;
+                    pending[p][k] = false;
+                }
+    }
 
     ~ssasdpd_engine() { release(); }
 
@@ -148,6 +182,12 @@ struct ssasdpd_engine {
         dfree(rx); dfree(jcN); dfree(irN); dfree(prN); dfree(jcG); dfree(irG); dfree(Ndense); dfree(Dmat); dfree(bc);
         dfree(xx); dfree(rrate); dfree(srrate); dfree(sdrate); dfree(Ddiag); dfree(qtime); dfree(tmin); dfree(targ);
         dfree(qs); dfree(qwork); dfree(d_psum); dfree(d_words); dfree(d_expo); dfree(d_trace); dfree(derr);
+        for (int p = 0; p < NPH; p++)
+            for (int k = 0; k < 2; k++) {
+                if (ev_a[p][k]) (void)hipEventDestroy(ev_a[p][k]);
+                if (ev_b[p][k]) (void)hipEventDestroy(ev_b[p][k]);
+                ev_a[p][k] = ev_b[p][k] = nullptr;
+            }
         if (st) (void)hipStreamDestroy(st);
         st = nullptr;
     }
@@ -171,6 +211,11 @@ struct ssasdpd_engine {
         alpha = kernel_alpha(dim, h);
         out_steps.assign(d->output_steps, d->output_steps + d->n_output_steps);
         HIP_OK(hipStreamCreate(&st));
+        for (int p = 0; p < NPH; p++)
+            for (int k = 0; k < 2; k++) {
+                HIP_OK(hipEventCreate(&ev_a[p][k]));
+                HIP_OK(hipEventCreate(&ev_b[p][k]));
+            }
         size_t n3 = 3 * (size_t)n;
         x = dalloc<double>(n3); v = dalloc<double>(n3); vt = dalloc<double>(n3); F = dalloc<double>(n3);
         Fbp = dalloc<double>(n3); normal = dalloc<double>(n3); xtree = dalloc<double>(n3); spts = dalloc<double>(n3);
@@ -317,6 +362,7 @@ struct ssasdpd_engine {
         max_deg_seen = 0;
         memset(&stats, 0, sizeof(stats));
         stats.bad_particle = -1;
+        for (int p = 0; p < NPH; p++) { ph_ms[p] = 0; ph_launch[p] = 0; pending[p][0] = pending[p][1] = false; }
     }
 
     DevErr read_err() {
@@ -329,6 +375,7 @@ struct ssasdpd_engine {
     // Bin the tree points (= current x), and build the kd-tree when list
     // order needs it.
     void bin_tree_points() {
+        ph_begin(0);
         HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
         compute_bbox(st, xtree, n, dim, d_bbox, d_bbtmp);
         double bb[6];
@@ -349,11 +396,13 @@ struct ssasdpd_engine {
             kd_built = true;
         }
         if (!want_kd) kd.valid = false;
+        ph_end(0);
     }
 
     // Fixed-radius search from query = current x against the binned tree
     // points, then r and D_ij with the current rho (find_neighbors).
     void search() {
+        ph_begin(1);
         for (int attempt = 0; attempt < 4; attempt++) {
             HIP_OK(hipMemsetAsync(&derr->overflow, 0, sizeof(int), st));
             HIP_OK(hipMemsetAsync(&derr->max_deg, 0, sizeof(int), st));
@@ -365,6 +414,7 @@ struct ssasdpd_engine {
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
         finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr);
+        ph_end(1);
         lists_built = true;
     }
 
@@ -429,41 +479,48 @@ struct ssasdpd_engine {
     int do_step() {
         const unsigned step = cur;
         const bool moving = static_domain == 0;
-        auto t_start = std::chrono::steady_clock::now();
         if (step == 0 || moving) {
             bin_tree_points();
             search();                              // substep 0 lists (simulate.cpp:61-63)
         }
         SdpdArgs sa = sdpd_args(step);
+        ph_begin(2);
         sdpd_predict(st, sa);                      // substep 0
+        ph_end(2);
         if (step > 0 && moving) {
             search();                              // substep 1 lists (simulate.cpp:121-123)
             sa = sdpd_args(step);
         }
+        ph_begin(5);
         sdpd_forces(st, sa);                       // substep 1
+        ph_end(5);
+        ph_begin(2);
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);
-        HIP_OK(hipStreamSynchronize(st));
-        auto t_sdpd = std::chrono::steady_clock::now();
+        ph_end(2);
         int rc = 0;
         if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
             RdmeArgs ra = rdme_args(step);
             if (moving || !rdme_initialized) {
+                ph_begin(3);
                 rdme_rates(st, ra);
                 rdme_queue_build(st, ra, scan_tmp, qwork);
+                ph_end(3);
                 QueueState q;
                 d2h(st, &q, qs, 1);
                 HIP_OK(hipStreamSynchronize(st));
                 if (q.initialized == 1) rdme_initialized = true;
             }
-            auto t_rates = std::chrono::steady_clock::now();
-            stats.ms_rates += std::chrono::duration<double, std::milli>(t_rates - t_sdpd).count();
+            ph_begin(4);
             rdme_take_step(st, ra);
-            HIP_OK(hipStreamSynchronize(st));
-            stats.ms_ssa += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rates).count();
+            ph_end(4);
         }
         rc = check(step);
-        stats.ms_sdpd += std::chrono::duration<double, std::milli>(t_sdpd - t_start).count();
+        ph_collect();
+        stats.ms_neighbors = ph_ms[0] + ph_ms[1];
+        stats.ms_sdpd = ph_ms[2] + ph_ms[5];
+        stats.ms_rates = ph_ms[3];
+        stats.ms_ssa = ph_ms[4];
         cur++;
         stats.steps = cur;
         stats.particle_steps = (uint64_t)cur * n;
@@ -626,6 +683,14 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s) {
     *s = e->stats;
     return 0;
+}
+
+int ssasdpd_phase_times(ssasdpd_engine *e, double *ms, uint64_t *launches, int32_t n_phases) {
+    for (int p = 0; p < n_phases && p < ssasdpd_engine::NPH; p++) {
+        if (ms) ms[p] = e->ph_ms[p];
+        if (launches) launches[p] = e->ph_launch[p];
+    }
+    return ssasdpd_engine::NPH;
 }
 
 int ssasdpd_get_state(ssasdpd_engine *e, ssasdpd_state *s) {

@@ -93,6 +93,15 @@ class Engine:
         self.lib.ssasdpd_stats_get(self.h, C.byref(s))
         return s
 
+    PHASES = ("bin_tree", "search", "sdpd", "rates_queue", "ssa", "forces")
+
+    def phase_times(self):
+        """GPU ms and launch counts per phase since the last reset."""
+        ms = np.zeros(8)
+        nl = np.zeros(8, np.uint64)
+        k = self.lib.ssasdpd_phase_times(self.h, _p(ms, C.c_double), _p(nl, C.c_uint64), 8)
+        return {name: (float(ms[i]), int(nl[i])) for i, name in enumerate(self.PHASES[:k])}
+
     def state(self):
         n = self.sa.n
         S, R, Sc = self.S, self.R, self.Sc
