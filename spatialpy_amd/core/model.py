@@ -1,0 +1,304 @@
+"""Model description API mirroring spatialpy.core (Model, Species, Parameter,
+Reaction, TimeSpan, initial and boundary conditions).  Only what the engine's
+lowering needs is restated; the semantics follow the reference files cited
+in each docstring."""
+import math
+from collections import OrderedDict
+
+import numpy as np
+
+from .errors import ModelError
+
+
+class Species:
+    """spatialpy/core/species.py: name, diffusion coefficient, restrict_to."""
+
+    def __init__(self, name=None, diffusion_coefficient=None, restrict_to=None):
+        if name is None or not isinstance(name, str):
+            raise ModelError("Species name must be a string")
+        if diffusion_coefficient is None:
+            raise ModelError("Species diffusion_coefficient is required")
+        self.name = name
+        self.diffusion_coefficient = diffusion_coefficient
+        if restrict_to is not None and not isinstance(restrict_to, list):
+            restrict_to = [restrict_to]
+        self.restrict_to = None if restrict_to is None else [f"type_{t}" for t in restrict_to]
+
+    def __repr__(self):
+        return f"Species({self.name}, D={self.diffusion_coefficient})"
+
+
+class Parameter:
+    """spatialpy/core/parameter.py: a named value given as a number or an
+    expression over earlier parameters."""
+
+    def __init__(self, name=None, expression=None, value=None):
+        if name is None:
+            raise ModelError("Parameter name is required")
+        self.name = name
+        self.expression = expression if expression is not None else value
+        self.value = None if value is None else float(value)
+
+    def _evaluate(self, namespace):
+        if self.value is None or self.expression is not None:
+            expr = self.expression
+            if isinstance(expr, (int, float)):
+                self.value = float(expr)
+            else:
+                self.value = float(eval(str(expr), {"__builtins__": {}, "math": math},
+                                        dict(namespace)))
+        return self.value
+
+
+class Reaction:
+    """spatialpy/core/reaction.py.  Mass-action reactions only (rate given);
+    the propensity expression text follows Reaction._create_mass_action
+    (reaction.py:260-313)."""
+
+    def __init__(self, name=None, reactants=None, products=None, propensity_function=None,
+                 ode_propensity_function=None, rate=None, annotation=None, restrict_to=None):
+        self.name = name
+        self.reactants = OrderedDict(reactants or {})
+        self.products = OrderedDict(products or {})
+        self.marate = rate
+        self.annotation = annotation
+        self.massaction = propensity_function is None
+        self.propensity_function = propensity_function
+        self.ode_propensity_function = ode_propensity_function
+        if restrict_to is not None and not isinstance(restrict_to, list):
+            restrict_to = [restrict_to]
+        self.restrict_to = restrict_to
+        if self.massaction and rate is None:
+            raise ModelError(f"Reaction {name}: give either rate (mass action) or propensity_function")
+        order = sum(self.reactants.values())
+        if self.massaction and order > 2:
+            raise ModelError("A mass-action reaction cannot involve more than two of one species "
+                             "or one of two species")
+
+    def propensity_expression(self):
+        """The C expression SpatialPy generates for this mass-action
+        reaction (reaction.py:260-313 through the expression converter)."""
+        k = self.marate.name if isinstance(self.marate, Parameter) else str(self.marate)
+        names = [(s.name if isinstance(s, Species) else s, c) for s, c in self.reactants.items()]
+        if not names:
+            return f"({k}*vol)"
+        if len(names) == 1 and names[0][1] == 2:
+            x = names[0][0]
+            return f"((({k}*{x})*({x}-1))/vol)"
+        if len(names) == 1:
+            return f"({k}*{names[0][0]})"
+        return f"((({k}*{names[0][0]})*{names[1][0]})/vol)"
+
+
+class TimeSpan:
+    """spatialpy/core/timespan.py: output times and the output steps
+    (TimeSpan._set_timesteps, timespan.py:72-85)."""
+
+    def __init__(self, items, timestep_size=None):
+        items = np.asarray(items, dtype=float)
+        if items.size < 2:
+            raise ModelError("TimeSpan needs at least two points")
+        diff = np.diff(items)
+        if not np.allclose(diff, diff[0]):
+            raise ModelError("TimeSpan items must be uniformly spaced")
+        self.timestep_size = timestep_size
+        self._set_timesteps(diff[0], len(diff))
+
+    def _set_timesteps(self, output_interval, num_steps):
+        if self.timestep_size is None:
+            self.timestep_size = output_interval
+        self.output_freq = output_interval / self.timestep_size
+        self.num_timesteps = math.ceil(num_steps * self.output_freq)
+        steps = np.arange(0, self.num_timesteps + self.timestep_size, self.output_freq)
+        self.output_steps = np.unique(np.round(steps).astype(int))
+        self.items = self.output_steps * self.timestep_size
+
+    @classmethod
+    def linspace(cls, t=20, num_points=None, timestep_size=None):
+        if num_points is None:
+            num_points = int(t / 0.05) + 1
+        return cls(np.linspace(0, t, num_points), timestep_size=timestep_size)
+
+    @classmethod
+    def arange(cls, increment, t=20, timestep_size=None):
+        return cls(np.arange(0, t + increment, increment), timestep_size=timestep_size)
+
+    def __len__(self):
+        return len(self.items)
+
+    def __iter__(self):
+        return iter(self.items)
+
+    def __getitem__(self, k):
+        return self.items[k]
+
+
+class PlaceInitialCondition:
+    """initialcondition.py:35-75: count molecules at the vertex closest to location."""
+
+    def __init__(self, species, count, location=None):
+        self.species, self.count, self.location = species, count, location if location is not None else [0, 0, 0]
+
+    def apply(self, model):
+        s = model._species_index(self.species)
+        vtx = int(np.argmin(np.sum((model.domain.vertices - np.asarray(self.location)) ** 2, axis=1)))
+        model.u0[s, vtx] += self.count
+
+
+class UniformInitialCondition:
+    """initialcondition.py:77-130: count molecules in every voxel (of types)."""
+
+    def __init__(self, species, count, types=None):
+        self.species, self.count = species, count
+        self.types = None if types is None else [f"type_{t}" for t in types]
+
+    def apply(self, model):
+        s = model._species_index(self.species)
+        for i, t in enumerate(model.domain.type_id):
+            if self.types is None or t in self.types:
+                model.u0[s, i] += self.count
+
+
+class ScatterInitialCondition:
+    """initialcondition.py:133-200: count molecules on uniformly random voxels
+    (of types).  The reference draws from the global numpy.random; a seed
+    here (or a Generator) makes it reproducible."""
+
+    def __init__(self, species, count, types=None, seed=None):
+        self.species, self.count = species, count
+        self.types = None if types is None else [f"type_{t}" for t in types]
+        self.seed = seed
+
+    def apply(self, model):
+        s = model._species_index(self.species)
+        if self.types is None:
+            allowed = np.arange(model.domain.get_num_voxels())
+        else:
+            allowed = np.array([i for i, t in enumerate(model.domain.type_id) if t in self.types])
+            if allowed.size == 0:
+                raise ModelError("ScatterInitialCondition has no voxel of the given types")
+        rng = np.random.default_rng(self.seed) if self.seed is not None else np.random
+        idx = allowed[rng.integers(0, allowed.size, size=self.count)] if self.seed is not None else \
+            allowed[np.random.randint(0, allowed.size, size=self.count)]
+        np.add.at(model.u0[s], idx, 1)
+
+
+class BoundaryCondition:
+    """boundarycondition.py: a box (any bound may be None), an optional type,
+    and a target (deterministic species, 'v', 'nu' or 'rho') set to value."""
+
+    def __init__(self, xmin=None, xmax=None, ymin=None, ymax=None, zmin=None, zmax=None,
+                 type_id=None, deterministic=True, target=None, value=None, model=None):
+        if target is None:
+            raise ModelError("BoundaryCondition needs a target")
+        self.bounds = (xmin, xmax, ymin, ymax, zmin, zmax)
+        self.type_id = None if type_id is None else f"type_{type_id}"
+        self.deterministic = deterministic
+        self.target = target.name if isinstance(target, Species) else target
+        if isinstance(value, int):
+            value = float(value)
+        self.value = value
+        if all(b is None for b in self.bounds) and self.type_id is None:
+            raise ModelError("need at least one condition on the BoundaryCondition")
+
+
+class Model:
+    """spatialpy/core/model.py:65-1056 (the parts the engine consumes)."""
+
+    def __init__(self, name="spatialpy"):
+        self.name = name
+        self.listOfParameters = OrderedDict()
+        self.listOfSpecies = OrderedDict()
+        self.listOfReactions = OrderedDict()
+        self.listOfInitialConditions = []
+        self.listOfBoundaryConditions = []
+        self.listOfDataFunctions = OrderedDict()
+        self.domain = None
+        self.staticDomain = True
+        self.enable_rdme = True
+        self.enable_pde = True
+        self.tspan = None
+        self.u0 = None
+
+    # -- building
+    def add_species(self, species):
+        for s in species if isinstance(species, list) else [species]:
+            self.listOfSpecies[s.name] = s
+        return species
+
+    def add_parameter(self, params):
+        for p in params if isinstance(params, list) else [params]:
+            self.listOfParameters[p.name] = p
+        return params
+
+    def add_reaction(self, reactions):
+        for r in reactions if isinstance(reactions, list) else [reactions]:
+            self.listOfReactions[r.name] = r
+        return reactions
+
+    def add_domain(self, domain, allow_all_types=False):
+        self.domain = domain
+
+    def add_initial_condition(self, ic):
+        self.listOfInitialConditions.append(ic)
+
+    def add_boundary_condition(self, bc):
+        self.listOfBoundaryConditions.append(bc)
+
+    def add_data_function(self, df):
+        raise ModelError("DataFunctions are not supported by this engine yet")
+
+    def timespan(self, time_span, timestep_size=None):
+        if isinstance(time_span, TimeSpan):
+            self.tspan = time_span
+        else:
+            self.tspan = TimeSpan(time_span, timestep_size=timestep_size)
+
+    def set_timesteps(self, output_interval, num_steps, timestep_size=None):
+        self.tspan = TimeSpan(np.arange(0, (num_steps + 1) * output_interval, output_interval)[: num_steps + 1],
+                              timestep_size=timestep_size)
+
+    def get_num_species(self):
+        return len(self.listOfSpecies)
+
+    def get_num_reactions(self):
+        return len(self.listOfReactions)
+
+    def _species_index(self, sp):
+        name = sp.name if isinstance(sp, Species) else sp
+        if name not in self.listOfSpecies:
+            raise ModelError(f"Species {name} does not exist in the model.")
+        return list(self.listOfSpecies).index(name)
+
+    # -- lowering helpers
+    def resolve_parameters(self):
+        ns = OrderedDict()
+        for name, p in self.listOfParameters.items():
+            ns[name] = p._evaluate(ns)
+        return ns
+
+    def compile_prep(self):
+        """Validate and fill u0[species, voxel] (model.py:178-185, 982-1019)."""
+        if self.tspan is None:
+            raise ModelError("The model's timespan is not set.")
+        if self.domain is None:
+            raise ModelError("The model's domain is not set.  Use 'add_domain()'.")
+        self.domain.compile_prep()
+        self.u0 = np.zeros((self.get_num_species(), self.domain.get_num_voxels()))
+        for ic in self.listOfInitialConditions:
+            ic.apply(self)
+        return self.resolve_parameters()
+
+    def run(self, number_of_trajectories=1, seed=None, timeout=None, number_of_threads=None,
+            debug_level=0, debug=False, profile=False, solver=None):
+        """model.py:1021-1056 plus `solver=`: a Solver class or instance
+        (defaults to the GPU Solver)."""
+        from ..solver import Solver
+        if solver is None:
+            sol = Solver(self, debug_level=debug_level)
+        elif isinstance(solver, type):
+            sol = solver(self, debug_level=debug_level)
+        else:
+            sol = solver
+        return sol.run(number_of_trajectories=number_of_trajectories, seed=seed, timeout=timeout,
+                       number_of_threads=number_of_threads, debug=debug, profile=profile)

@@ -107,8 +107,10 @@ struct ssasdpd_engine {
     // RDME state
     uint32_t *xx = nullptr;
     double *rrate = nullptr, *srrate = nullptr, *sdrate = nullptr, *Ddiag = nullptr, *qtime = nullptr, *tmin = nullptr;
-    int *targ = nullptr;
-    int nlev = 0, lev_off[8] = {0}, lev_n[8] = {0};
+    double *vol = nullptr;
+    int *targ = nullptr, *ntype = nullptr;
+    int pl = 1, nlev = 0, lev_off[10] = {0}, lev_n[10] = {0}, tree_nodes = 0;
+    size_t lds_bytes = 0;
     QueueState *qs = nullptr;
     int *qwork = nullptr;
     double *d_psum = nullptr;
@@ -171,7 +173,7 @@ struct ssasdpd_engine {
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
         dfree(rho); dfree(old_rho); dfree(Frho); dfree(mass); dfree(nu); dfree(bvf);
         dfree(type); dfree(solid); dfree(C); dfree(Q);
-        dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij);
+        dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(vol);
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
         dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
         dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
@@ -193,10 +195,11 @@ struct ssasdpd_engine {
     }
 
     void alloc_lists(int w) {
-        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij);
+        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype);
         W = w;
         size_t cap = (size_t)W * n;
         nidx = dalloc<int>(cap);
+        ntype = dalloc<int>(cap);
         nr2 = dalloc<double>(cap);
         nr = dalloc<double>(cap);
         Dij = dalloc<double>(cap);
@@ -279,21 +282,35 @@ struct ssasdpd_engine {
             srrate = dalloc<double>(n); sdrate = dalloc<double>(n);
             Ddiag = dalloc<double>((size_t)n * S);
             qtime = dalloc<double>(n);
-            // 64-ary min-tree levels
+            // event tree: level 1 groups F0 = 64*pl voxels (pl chosen so the
+            // levels above the voxels fit the event loop's LDS), 64 per node above
+            pl = 1;
+            while (pl < 32 && (n + 64 * pl - 1) / (64 * pl) > 3072) pl *= 2;
+            if ((n + 64 * pl - 1) / (64 * pl) > 3072)
+                throw HipError("too many voxels for the event tree (max 6.2M)", hipErrorUnknown, __FILE__, __LINE__);
             lev_n[0] = n;
-            nlev = 0;
-            int tot = 0;
+            lev_n[1] = (n + 64 * pl - 1) / (64 * pl);
+            lev_off[1] = 0;
+            nlev = 1;
+            tree_nodes = lev_n[1];
             while (lev_n[nlev] > 1) {
-                lev_off[nlev] = tot;
                 lev_n[nlev + 1] = (lev_n[nlev] + TREE_FAN - 1) / TREE_FAN;
-                tot += lev_n[nlev + 1];
+                lev_off[nlev + 1] = tree_nodes;
+                tree_nodes += lev_n[nlev + 1];
                 nlev++;
-                if (nlev >= 7) throw HipError("too many voxels for the event tree", hipErrorUnknown, __FILE__, __LINE__);
             }
-            tmin = dalloc<double>(std::max(tot, 1));
-            targ = dalloc<int>(std::max(tot, 1));
+            tmin = dalloc<double>(tree_nodes);
+            targ = dalloc<int>(tree_nodes);
+            vol = dalloc<double>(n);
+            {
+                int nnzN = R > 0 ? d->jcN[R] : 0, nnzG = d->jcG[S + R];
+                lds_bytes = ssa_lds_bytes(tree_nodes, S, R, T, nnzN, nnzG);
+                if (lds_bytes > 65536)
+                    throw HipError("event-loop constants exceed 64 KiB of LDS", hipErrorUnknown, __FILE__, __LINE__);
+            }
             qs = dalloc<QueueState>(1);
-            qwork = (int *)dalloc<char>(sizeof(double) * (size_t)n + sizeof(int) * 4 * ((size_t)n + 1) + 64);
+            qwork = (int *)dalloc<char>(sizeof(double) * (size_t)n + sizeof(int) * 4 * ((size_t)n + 1) + 64 +
+                                        sizeof(double) * 512);
         }
         if (Sc > 0 && Rc > 0 && d->N_dense) {
             Ndense = dalloc<int>((size_t)Sc * Rc);
@@ -413,7 +430,7 @@ struct ssasdpd_engine {
             if (!e.overflow) break;
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
-        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr);
+        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr, type, ntype);
         ph_end(1);
         lists_built = true;
     }
@@ -434,9 +451,10 @@ struct ssasdpd_engine {
         a.n = n; a.S = S; a.R = R; a.T = T; a.W = W;
         a.mass = mass; a.rho = rho; a.type = type; a.xx = xx; a.rrate = rrate; a.srrate = srrate; a.sdrate = sdrate;
         a.Ddiag = Ddiag; a.Dmat = Dmat; a.rx = rx; a.jcN = jcN; a.irN = irN; a.prN = prN; a.jcG = jcG; a.irG = irG;
-        a.deg = deg; a.nidx = nidx; a.Dij = Dij;
-        a.qtime = qtime; a.tmin = tmin; a.targ = targ; a.nlev = nlev;
-        for (int l = 0; l < 8; l++) { a.lev_off[l] = lev_off[l]; a.lev_n[l] = lev_n[l]; }
+        a.deg = deg; a.nidx = nidx; a.ntype = ntype; a.Dij = Dij; a.vol = vol;
+        a.qtime = qtime; a.tmin = tmin; a.targ = targ; a.nlev = nlev; a.pl = pl; a.tree_nodes = tree_nodes;
+        a.lds_bytes = lds_bytes;
+        for (int l = 0; l < 10; l++) { a.lev_off[l] = lev_off[l]; a.lev_n[l] = lev_n[l]; }
         a.qs = qs;
         a.rng_kind = rng_kind; a.seed = seed; a.stream_id = 0; a.words = d_words; a.expo = d_expo; a.n_words = n_words;
         a.textbook = (flags & SSASDPD_FLAG_TEXTBOOK_SELECTION) ? 1 : 0;

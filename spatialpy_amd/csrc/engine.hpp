@@ -111,7 +111,7 @@ void exclusive_scan_i32(hipStream_t s, const int *in, int *out, long long n, voi
 // D_ij and r from r2 (add_to_neighbor_list, ENG/src/particle.cpp:150-210).
 void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2,
                   double *r, double *Dij, const double *mass, const double *rho, double h,
-                  DevErr *err);
+                  DevErr *err, const int *type, int *ntype);
 
 // SDPD substeps.
 struct SdpdArgs {
@@ -147,15 +147,22 @@ struct RdmeArgs {
     const double *Dmat;
     const Rxn *rx;
     const int *jcN, *irN, *prN, *jcG, *irG;
-    const int *deg, *nidx;
+    const int *deg, *nidx, *ntype;
     const double *Dij;
-    // queue
-    double *qtime;           // level 0
-    double *tmin;            // upper levels, concatenated
+    double *vol;             // mass/rho per voxel, set by the rates kernel
+    // queue: level 0 = firing times of the voxels; level k >= 1 holds the
+    // (time, voxel) minimum of its children: F0 = 64*pl voxels per level-1
+    // node, 64 nodes per node above.  Levels 1..nlev live in tmin/targ at
+    // lev_off[k]; level nlev is the root.
+    double *qtime;
+    double *tmin;
     int *targ;
-    int nlev;                // number of upper levels
-    int lev_off[8];          // offset of each upper level in tmin/targ
-    int lev_n[8];            // nodes per level (level 0 = n)
+    int pl;                  // level-0 children per lane (F0 = 64*pl)
+    int nlev;                // number of levels above level 0 (>= 1)
+    int lev_off[10];
+    int lev_n[10];           // nodes per level (lev_n[0] = n)
+    int tree_nodes;          // sum of lev_n[1..nlev]
+    size_t lds_bytes;        // dynamic LDS of the event-loop kernel
     QueueState *qs;
     // random stream
     int rng_kind;            // 0 seeded (Philox), 1 replay
@@ -177,5 +184,7 @@ struct RdmeArgs {
 void rdme_rates(hipStream_t s, const RdmeArgs &a);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
 void rdme_take_step(hipStream_t s, const RdmeArgs &a);
+void tree_rebuild(hipStream_t s, const RdmeArgs &a);
+size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG);
 
 }  // namespace ssa

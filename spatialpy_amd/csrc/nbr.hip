@@ -133,19 +133,29 @@ __global__ void k_bbox_init(unsigned long long *t) {
     if (i < 3) { t[i] = ~0ULL; t[3 + i] = 0ULL; }
 }
 __global__ void k_bbox(const double *pts, int n, int dim, unsigned long long *t) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long smin[3][4], smax[3][4];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     for (int d = 0; d < dim; d++) {
         unsigned long long kmin = ~0ULL, kmax = 0;
-        if (i < n) { kmin = kmax = dkey(pts[(size_t)d * n + i]); }
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            unsigned long long k = dkey(pts[(size_t)d * n + i]);
+            kmin = k < kmin ? k : kmin;
+            kmax = k > kmax ? k : kmax;
+        }
         for (int o = WAVE / 2; o > 0; o >>= 1) {
             unsigned long long a = __shfl_down(kmin, o, WAVE), b = __shfl_down(kmax, o, WAVE);
             kmin = a < kmin ? a : kmin;
             kmax = b > kmax ? b : kmax;
         }
-        if ((threadIdx.x & (WAVE - 1)) == 0) {
-            atomicMin(&t[d], kmin);
-            atomicMax(&t[3 + d], kmax);
-        }
+        if (lane == 0) { smin[d][w] = kmin; smax[d][w] = kmax; }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)dim) {
+        int d = threadIdx.x;
+        unsigned long long a = smin[d][0], b = smax[d][0];
+        for (int k = 1; k < 4; k++) { a = smin[d][k] < a ? smin[d][k] : a; b = smax[d][k] > b ? smax[d][k] : b; }
+        atomicMin(&t[d], a);
+        atomicMax(&t[3 + d], b);
     }
 }
 __global__ void k_bbox_fin(const unsigned long long *t, double *bb, int dim) {
@@ -157,7 +167,8 @@ __global__ void k_bbox_fin(const unsigned long long *t, double *bb, int dim) {
 }
 void compute_bbox(hipStream_t s, const double *pts, int n, int dim, double *d_bbox, unsigned long long *tmp) {
     hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, tmp);
-    hipLaunchKernelGGL(k_bbox, dim3((n + 255) / 256), dim3(256), 0, s, pts, n, dim, tmp);
+    unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 512);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, s, pts, n, dim, tmp);
     hipLaunchKernelGGL(k_bbox_fin, dim3(1), dim3(64), 0, s, tmp, d_bbox, dim);
 }
 
@@ -675,7 +686,8 @@ void fr_search(hipStream_t s, const double *q, int m, const double *spts, const 
 // ---------------------------------------------------------------------------
 // r = sqrt(r2) and D_ij per edge (add_to_neighbor_list, particle.cpp:160-207).
 __global__ void k_finish(int n, int W, const int *deg, const int *idx, const double *r2, double *r, double *Dij,
-                         const double *mass, const double *rho, double h, DevErr *err) {
+                         const double *mass, const double *rho, double h, DevErr *err, const int *type,
+                         int *ntype) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int dg = deg[i];
@@ -697,13 +709,15 @@ __global__ void k_finish(int n, int W, const int *deg, const int *idx, const dou
         }
         r[e] = rr;
         Dij[e] = D;
+        ntype[e] = type[j];
     }
 }
 
 void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2, double *r,
-                  double *Dij, const double *mass, const double *rho, double h, DevErr *err) {
+                  double *Dij, const double *mass, const double *rho, double h, DevErr *err, const int *type,
+                  int *ntype) {
     hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, s, n, W, deg, idx, r2, r, Dij, mass, rho, h,
-                       err);
+                       err, type, ntype);
 }
 
 }  // namespace ssa

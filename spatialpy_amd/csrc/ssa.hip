@@ -7,8 +7,10 @@
 //                     (simulate_rdme.cpp:155-195, NRMConstant_v5.cpp:27-85)
 //   rdme_take_step    nsm_core__take_step (simulate_rdme.cpp:211-472)
 //
-// The event queue is an indexed 64-ary min-tree over the voxels' firing
-// times (one wavefront reduces one node).  The reference's time-binned hash
+// The event queue is an indexed min-tree over the voxels' firing times:
+// F0 = 64*PL voxels under each level-1 node, 64 nodes under each node above
+// (one wavefront reduces one node).  The levels above the voxels live in LDS
+// while the event loop runs.  The reference's time-binned hash
 // table always yields the global minimum among firing times inside its
 // current window [L, U] and rebuilds the window when that set is empty
 // (NRMConstant_v5.cpp:156-281); the window scalars are kept here with the
@@ -113,6 +115,7 @@ __global__ void k_rates(RdmeArgs a) {
         sr += p;
     }
     a.srrate[i] = sr;
+    a.vol[i] = a.mass[i] / a.rho[i];
     double sd = 0.0;
     const int dg = a.deg[i];
     for (int s = 0; s < S; s++) {
@@ -167,10 +170,23 @@ __global__ void k_qprop(RdmeArgs a, double *prop, int *nz, int *act) {
     act[i] = p > 0;
 }
 
-// propensitySum in index order (simulate_rdme.cpp:167-179) by one wavefront:
-// each 64-value chunk is loaded by the lanes, then added in order.
-__global__ void k_qsum(RdmeArgs a, const double *prop, const int *act_scan, int exact) {
-    __shared__ double part[1];
+// Partial sums for the fast (seeded) mode: fixed block/lane order, so the
+// result is deterministic.
+constexpr int QSUM_BLOCKS = 256;
+__global__ void k_qsum_part(const double *prop, int n, double *part) {
+    __shared__ double w[4];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += prop[i];
+    for (int o = WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0) w[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+// propensitySum (simulate_rdme.cpp:167-179) and the first time window.
+// exact: added in voxel order by one wavefront (bit-identical to the
+// reference's loop); otherwise the block partials are added in order.
+__global__ void k_qsum(RdmeArgs a, const double *prop, const int *act_scan, int exact, const double *part) {
     const int lane = threadIdx.x;
     double s = 0.0;
     if (exact) {
@@ -180,11 +196,9 @@ __global__ void k_qsum(RdmeArgs a, const double *prop, const int *act_scan, int 
             for (int k = 0; k < m; k++) s += __shfl(v, k, WAVE);
         }
     } else {
-        for (int i = lane; i < a.n; i += WAVE) s += prop[i];
-        for (int o = WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
+        for (int k = 0; k < QSUM_BLOCKS; k++) s += part[k];
     }
     if (lane == 0) {
-        part[0] = s;
         QueueState q = *a.qs;
         q.endTime = a.endTime;
         long long active = act_scan[a.n];
@@ -231,8 +245,7 @@ __global__ void k_qfinish(RdmeArgs a, const int *nz_scan) {
     *a.qs = q;
 }
 
-// One level of the min-tree: node j of the upper level = argmin of 64
-// children (ties -> lower voxel index).
+// Argmin of (time, voxel) pairs: ties go to the lower voxel index.
 __device__ __forceinline__ void argmin2(double &t, int &v, double t2, int v2) {
     if (t2 < t || (t2 == t && v2 < v)) { t = t2; v = v2; }
 }
@@ -244,6 +257,22 @@ __device__ __forceinline__ void wave_argmin(double &t, int &v) {
         argmin2(t, v, t2, v2);
     }
 }
+// level 1 from the firing times: one wavefront per node of F0 = 64*pl voxels
+__global__ void k_tree_level1(const double *qt, int n, int pl, double *pt, int *pa, int nparent) {
+    int node = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    int lane = threadIdx.x & (WAVE - 1);
+    if (node >= nparent) return;
+    double t = INFINITY;
+    int v = INT_MAX;
+    int base = node * WAVE * pl + lane * pl;
+    for (int p = 0; p < pl; p++) {
+        int c = base + p;
+        if (c < n) argmin2(t, v, qt[c], c);
+    }
+    wave_argmin(t, v);
+    if (lane == 0) { pt[node] = t; pa[node] = v; }
+}
+// upper level: 64 children per node
 __global__ void k_tree_level(const double *ct, const int *ca, int nchild, double *pt, int *pa, int nparent) {
     int node = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     int lane = threadIdx.x & (WAVE - 1);
@@ -251,79 +280,44 @@ __global__ void k_tree_level(const double *ct, const int *ca, int nchild, double
     int c = node * WAVE + lane;
     double t = INFINITY;
     int v = INT_MAX;
-    if (c < nchild) { t = ct[c]; v = ca ? ca[c] : c; }
+    if (c < nchild) { t = ct[c]; v = ca[c]; }
     wave_argmin(t, v);
     if (lane == 0) { pt[node] = t; pa[node] = v; }
 }
 
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *work) {
-    // work: prop (n doubles) | nz (n+1) | act (n+1) | nz_scan (n+1) | act_scan (n+1)
+    // work: prop (n doubles) | nz (n+1) | act (n+1) | nz_scan (n+1) | act_scan (n+1) | part
     double *prop = (double *)work;
     int *nz = (int *)(prop + a.n);
     int *act = nz + a.n + 1;
     int *nz_scan = act + a.n + 1;
     int *act_scan = nz_scan + a.n + 1;
+    double *part = (double *)(((uintptr_t)(act_scan + a.n + 1) + 15) & ~(uintptr_t)15);
     unsigned g = (unsigned)((a.n + 255) / 256);
+    const bool exact = a.rng_kind == 1;
     hipLaunchKernelGGL(k_qprop, dim3(g), dim3(256), 0, s, a, prop, nz, act);
     exclusive_scan_i32(s, nz, nz_scan, a.n, scan_tmp);
     exclusive_scan_i32(s, act, act_scan, a.n, scan_tmp);
-    hipLaunchKernelGGL(k_qsum, dim3(1), dim3(WAVE), 0, s, a, prop, act_scan, a.rng_kind == 1 ? 1 : 0);
+    if (!exact) hipLaunchKernelGGL(k_qsum_part, dim3(QSUM_BLOCKS), dim3(256), 0, s, prop, a.n, part);
+    hipLaunchKernelGGL(k_qsum, dim3(1), dim3(WAVE), 0, s, a, prop, act_scan, exact ? 1 : 0, part);
     hipLaunchKernelGGL(k_qtimes, dim3(g), dim3(256), 0, s, a, prop, nz_scan);
     hipLaunchKernelGGL(k_qfinish, dim3(1), dim3(64), 0, s, a, nz_scan);
-    // min-tree levels
-    const double *ct = a.qtime;
-    const int *ca = nullptr;
-    int nchild = a.n;
-    for (int l = 0; l < a.nlev; l++) {
-        int np = a.lev_n[l + 1];
-        double *pt = a.tmin + a.lev_off[l];
-        int *pa = a.targ + a.lev_off[l];
-        hipLaunchKernelGGL(k_tree_level, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, s, ct, ca, nchild, pt, pa, np);
-        ct = pt;
-        ca = pa;
-        nchild = np;
+    tree_rebuild(s, a);
+}
+
+void tree_rebuild(hipStream_t s, const RdmeArgs &a) {
+    int n1 = a.lev_n[1];
+    hipLaunchKernelGGL(k_tree_level1, dim3((unsigned)((n1 + 3) / 4)), dim3(256), 0, s, a.qtime, a.n, a.pl,
+                       a.tmin + a.lev_off[1], a.targ + a.lev_off[1], n1);
+    for (int k = 1; k < a.nlev; k++) {
+        int np = a.lev_n[k + 1];
+        hipLaunchKernelGGL(k_tree_level, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, s, a.tmin + a.lev_off[k],
+                           a.targ + a.lev_off[k], a.lev_n[k], a.tmin + a.lev_off[k + 1], a.targ + a.lev_off[k + 1], np);
     }
 }
 
 // ---------------------------------------------------------------------------
 // The event loop.
-struct Tree {
-    double *qtime, *tmin;
-    int *targ;
-    int n, nlev;
-    int lev_off[8], lev_n[8];
-
-    __device__ void root(double &t, int &v) const {
-        if (nlev == 0) { t = qtime[0]; v = 0; return; }
-        t = tmin[lev_off[nlev - 1]];
-        v = targ[lev_off[nlev - 1]];
-    }
-    // set the firing time of voxel v and repair the path to the root
-    __device__ void update(int v, double t, int lane) {
-        if (lane == 0) qtime[v] = t;
-        double ct = t;
-        int cv = v;
-        int idx = v;
-        for (int l = 0; l < nlev; l++) {
-            int node = idx / WAVE;
-            int c = node * WAVE + lane;
-            int nchild = lev_n[l];
-            double tt = INFINITY;
-            int vv = INT_MAX;
-            if (c < nchild) {
-                if (c == idx) { tt = ct; vv = cv; }
-                else if (l == 0) { tt = qtime[c]; vv = c; }
-                else { tt = tmin[lev_off[l - 1] + c]; vv = targ[lev_off[l - 1] + c]; }
-            }
-            wave_argmin(tt, vv);
-            if (lane == 0) { tmin[lev_off[l] + node] = tt; targ[lev_off[l] + node] = vv; }
-            ct = tt;
-            cv = vv;
-            idx = node;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-};
 
 // NRMConstant_v5::rebuild (NRMConstant_v5.cpp:194-242): returns false when
 // the window cannot move (selectReaction then reports an empty queue).
@@ -336,19 +330,230 @@ __device__ bool rebuild(QueueState &q) {
     return q.nbins > 0;
 }
 
+// LDS image of the model constants and the upper tree levels.
+struct Lds {
+    double *lt;     // upper-level times (offsets lev_off)
+    int *lv;        // upper-level voxels
+    Rxn *rx;
+    int *jcN, *irN, *prN, *jcG, *irG;
+    double *Dmat;
+};
+
+__device__ __forceinline__ size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Cross-lane moves through DPP (no LDS round trip).  Lanes whose source is
+// outside the pattern keep their own value.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, RMASK, 0xF, false);
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_d(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(dpp_i<CTRL, RMASK>(hi), dpp_i<CTRL, RMASK>(lo));
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void dpp_step(double &t, int &v) {
+    double t2 = dpp_d<CTRL, RMASK>(t);
+    int v2 = dpp_i<CTRL, RMASK>(v);
+    argmin2(t, v, t2, v2);
+}
+__device__ __forceinline__ double rl_d(double x, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+// argmin over the wavefront; every lane gets the result (ties -> lower voxel)
+__device__ __forceinline__ void wave_argmin_dpp(double &t, int &v) {
+    dpp_step<0xB1, 0xF>(t, v);    // quad_perm [1,0,3,2]
+    dpp_step<0x4E, 0xF>(t, v);    // quad_perm [2,3,0,1]
+    dpp_step<0x141, 0xF>(t, v);   // row_half_mirror
+    dpp_step<0x140, 0xF>(t, v);   // row_mirror
+    dpp_step<0x142, 0xA>(t, v);   // row_bcast15 into rows 1, 3
+    dpp_step<0x143, 0xC>(t, v);   // row_bcast31 into rows 2, 3
+    t = rl_d(t, 63);
+    v = __builtin_amdgcn_readlane(v, 63);
+}
+
+// Small per-voxel vectors are held one element per lane ("lane s holds
+// species s"); an element at a uniform index is read with v_readlane.
+__device__ __forceinline__ uint32_t rl_u(uint32_t x, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+
+// Mass-action propensity with the counts distributed over the lanes.
+__device__ __forceinline__ double propensity_l(const Rxn &re, uint32_t x_l, double vol, int tp) {
+    if (re.mask && !((re.mask >> tp) & 1ULL)) return 0.0;
+    const int xa = (int)rl_u(x_l, re.a);
+    switch (re.kind) {
+    case 0: return re.k * vol;
+    case 1: return re.k * xa;
+    case 2: return ((re.k * xa) * (int)rl_u(x_l, re.b)) / vol;
+    default: return ((re.k * xa) * (xa - 1)) / vol;
+    }
+}
+
+// Level-0 children of one level-1 node held in registers: lane l owns the
+// voxels base + l*PL .. base + l*PL + PL-1.
+template <int PL> struct Sib {
+    double t[PL];
+    int base;
+    __device__ __forceinline__ void load(const double *qt, int n, int node, int lane) {
+        base = node * WAVE * PL;
+#pragma unroll
+        for (int p = 0; p < PL; p++) {
+            int c = base + lane * PL + p;
+            t[p] = c < n ? qt[c] : INFINITY;
+        }
+    }
+    __device__ __forceinline__ bool holds(int v) const { return v >= base && v < base + WAVE * PL; }
+    __device__ __forceinline__ void set(int v, double tv, int lane) {
+        int off = v - base;
+#pragma unroll
+        for (int p = 0; p < PL; p++)
+            if (lane * PL + p == off) t[p] = tv;
+    }
+    __device__ __forceinline__ void argmin(double &m, int &am, int lane, int n) const {
+        m = INFINITY;
+        am = INT_MAX;
+#pragma unroll
+        for (int p = 0; p < PL; p++) {
+            int c = base + lane * PL + p;
+            if (c < n) argmin2(m, am, t[p], c);
+        }
+        wave_argmin_dpp(m, am);
+    }
+};
+
+// Repair the levels above a changed level-1 node value (idx -> (m, am)),
+// rescanning a node only when its stored minimum may have increased.
+template <int PL>
+__device__ void tree_fix_up(const RdmeArgs &a, Lds &L, int idx, double m, int am, double old_m, int old_am,
+                            int lane) {
+    for (int k = 1; k < a.nlev; k++) {
+        const int p = idx / WAVE;
+        const int poff = a.lev_off[k + 1] + p;
+        const double pm = L.lt[poff];
+        const int pam = L.lv[poff];
+        double nm;
+        int nam;
+        if (pam == old_am && (m > pm || (m == pm && am != pam))) {
+            // the parent's argmin came from this child and got worse: rescan
+            int c = p * WAVE + lane;
+            double tt = INFINITY;
+            int vv = INT_MAX;
+            if (c < a.lev_n[k]) {
+                if (c == idx) { tt = m; vv = am; }
+                else { tt = L.lt[a.lev_off[k] + c]; vv = L.lv[a.lev_off[k] + c]; }
+            }
+            wave_argmin_dpp(tt, vv);
+            nm = tt;
+            nam = vv;
+        } else {
+            nm = pm;
+            nam = pam;
+            argmin2(nm, nam, m, am);
+        }
+        if (nm == pm && nam == pam) return;   // nothing above changes
+        if (lane == 0) { L.lt[poff] = nm; L.lv[poff] = nam; }
+        old_m = pm;
+        old_am = pam;
+        m = nm;
+        am = nam;
+        idx = p;
+    }
+}
+
+// Voxel v gets time tv; its level-0 node is in sib (already holding tv).
+template <int PL>
+__device__ void tree_update_rescan(const RdmeArgs &a, Lds &L, const Sib<PL> &sib, int v, double tv, int lane) {
+    if (lane == 0) a.qtime[v] = tv;
+    double m;
+    int am;
+    sib.argmin(m, am, lane, a.n);
+    const int idx = v / (WAVE * PL);
+    const int off = a.lev_off[1] + idx;
+    const double om = L.lt[off];
+    const int oam = L.lv[off];
+    if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
+    if (!(m == om && am == oam)) tree_fix_up<PL>(a, L, idx, m, am, om, oam, lane);
+}
+
+// Voxel v gets time tv without its level-0 node in registers: the node's
+// minimum is updated in place unless v was the node's argmin and got later,
+// in which case the node is reloaded and rescanned.
+template <int PL>
+__device__ void tree_update_point(const RdmeArgs &a, Lds &L, int v, double tv, int lane) {
+    if (lane == 0) a.qtime[v] = tv;
+    const int idx = v / (WAVE * PL);
+    const int off = a.lev_off[1] + idx;
+    const double om = L.lt[off];
+    const int oam = L.lv[off];
+    double m;
+    int am;
+    if (oam == v && tv > om) {
+        Sib<PL> sb;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        sb.load(a.qtime, a.n, idx, lane);
+        sb.set(v, tv, lane);
+        sb.argmin(m, am, lane, a.n);
+    } else if (oam == v) {
+        m = tv;
+        am = v;
+    } else {
+        m = om;
+        am = oam;
+        argmin2(m, am, tv, v);
+    }
+    if (m == om && am == oam) return;
+    if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
+    tree_fix_up<PL>(a, L, idx, m, am, om, oam, lane);
+}
+
+template <int PL, int MS, int MR>
 __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
     QueueState q = *a.qs;
     if (q.initialized != 1 || q.err) return;
     const int S = a.S, R = a.R, n = a.n, T = a.T;
-    Tree tr;
-    tr.qtime = a.qtime; tr.tmin = a.tmin; tr.targ = a.targ; tr.n = n; tr.nlev = a.nlev;
-    for (int l = 0; l < 8; l++) { tr.lev_off[l] = a.lev_off[l]; tr.lev_n[l] = a.lev_n[l]; }
+
+    // ---- LDS image
+    Lds L;
+    uint32_t *nbx;   // prefetched neighbour counts [64][MS]
+    double *nbr;     // prefetched neighbour reaction rates [64][MR]
+    {
+        size_t off = 0;
+        L.lt = (double *)(smem + off); off = al16(off + sizeof(double) * a.tree_nodes);
+        L.lv = (int *)(smem + off); off = al16(off + sizeof(int) * a.tree_nodes);
+        L.rx = (Rxn *)(smem + off); off = al16(off + sizeof(Rxn) * R);
+        L.Dmat = (double *)(smem + off); off = al16(off + sizeof(double) * S * T);
+        int nnzN = R > 0 ? a.jcN[R] : 0, nnzG = a.jcG[S + R];
+        L.jcN = (int *)(smem + off); off = al16(off + sizeof(int) * (R + 1));
+        L.irN = (int *)(smem + off); off = al16(off + sizeof(int) * nnzN);
+        L.prN = (int *)(smem + off); off = al16(off + sizeof(int) * nnzN);
+        L.jcG = (int *)(smem + off); off = al16(off + sizeof(int) * (S + R + 1));
+        L.irG = (int *)(smem + off); off = al16(off + sizeof(int) * nnzG);
+        nbx = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * WAVE * MS);
+        nbr = (double *)(smem + off); off = al16(off + sizeof(double) * WAVE * MR);
+        for (int i = lane; i < a.tree_nodes; i += WAVE) { L.lt[i] = a.tmin[i]; L.lv[i] = a.targ[i]; }
+        for (int i = lane; i < R; i += WAVE) L.rx[i] = a.rx[i];
+        for (int i = lane; i < S * T; i += WAVE) L.Dmat[i] = a.Dmat[i];
+        for (int i = lane; i <= R && R > 0; i += WAVE) L.jcN[i] = a.jcN[i];
+        for (int i = lane; i < nnzN; i += WAVE) { L.irN[i] = a.irN[i]; L.prN[i] = a.prN[i]; }
+        for (int i = lane; i <= S + R; i += WAVE) L.jcG[i] = a.jcG[i];
+        for (int i = lane; i < nnzG; i += WAVE) L.irG[i] = a.irG[i];
+        __syncthreads();
+    }
+    const int root = a.lev_off[a.nlev];
+    const int W = a.W;
+
     Rng rng;
     rng.kind = a.rng_kind; rng.seed = a.seed; rng.stream = a.stream_id; rng.words = a.words; rng.expo = a.expo;
     rng.n_words = a.n_words; rng.pos = q.rng_pos; rng.exhausted = false;
     const bool textbook = a.textbook != 0;
     const bool lead = lane == 0;
+    const bool s_lane = lane < S, r_lane = lane < R;
 
     double tt = a.t0;
     const double end_time = a.t0 + a.dt;
@@ -360,12 +565,11 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
 
     while (tt <= end_time) {
         if (++guard > a.max_events) { err = SSASDPD_ERR_CAPACITY; break; }
-        // ---- selectReaction
-        double t;
-        int sv;
+        // ---- selectReaction: global minimum inside the window [L, U]
+        double t = L.lt[root];
+        int sv = L.lv[root];
         bool found = false;
         for (int it = 0; it < 4096; it++) {
-            tr.root(t, sv);
             if (sv != INT_MAX && t <= q.U) { found = true; break; }
             if (!rebuild(q)) break;
             if (!(t <= q.U)) break;   // window moved but still empty: empty queue this call
@@ -373,87 +577,105 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         if (!found) break;
         q.rxnCount++;
         tt = t;
-        const double vol = a.mass[sv] / a.rho[sv];
+        // ---- round trip 1: the voxel's state, its neighbour row and its
+        // level-0 node
+        Sib<PL> sib;
+        sib.load(a.qtime, n, sv / (WAVE * PL), lane);
+        const double vol = a.vol[sv];
         double sr = a.srrate[sv], sd = a.sdrate[sv];
-        double totrate = sr + sd;
         const int tsv = a.type[sv];
-        uint32_t xs[MAXS];
-        double Dd[MAXS], rr[MAXR];
-        for (int s = 0; s < S; s++) { xs[s] = a.xx[(size_t)sv * S + s]; Dd[s] = a.Ddiag[(size_t)sv * S + s]; }
-        for (int j = 0; j < R; j++) rr[j] = a.rrate[(size_t)sv * R + j];
+        const int dg = a.deg[sv];
+        uint32_t xs = s_lane ? a.xx[(size_t)sv * S + lane] : 0u;       // lane s: count of species s
+        const double Dd = s_lane ? a.Ddiag[(size_t)sv * S + lane] : 0.0;
+        double rr = r_lane ? a.rrate[(size_t)sv * R + lane] : 0.0;     // lane j: rate of reaction j
+        int nb_j = -1, nb_t = 1;
+        double nb_D = 0.0;
+        if (lane < W) {
+            const size_t e = (size_t)lane * n + sv;
+            nb_j = a.nidx[e];
+            nb_D = a.Dij[e];
+            if (T > 1) nb_t = a.ntype[e];
+        }
+        double totrate = sr + sd;
         const double rand1 = rng.uniform();
         if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
         int event;
         int which;
-        double dest_sr = 0.0, dest_sd = 0.0;
         if (rand1 <= sr / totrate) {
             // ---- reaction (simulate_rdme.cpp:255-310)
             event = 0;
             double rv = textbook ? rand1 * totrate : rand1 * sr;
             int re = 0;
-            double cum = R > 0 ? rr[0] : 0.0;
-            while (re < R && rv > cum) { re++; cum += re < R ? rr[re] : 0.0; }
+            double cum = R > 0 ? rl_d(rr, 0) : 0.0;
+            while (re < R && rv > cum) { re++; cum += re < R ? rl_d(rr, re) : 0.0; }
             if (re >= R) {
                 if (cum != sr) {
                     double rd = 0.0;
-                    for (int j = 0; j < R; j++) { rr[j] = propensity(a.rx[j], xs, vol, tsv); rd += rr[j]; }
-                    sr = rd;
-                    if (lead) {
-                        for (int j = 0; j < R; j++) a.rrate[(size_t)sv * R + j] = rr[j];
-                        a.srrate[sv] = sr;
+                    for (int j = 0; j < R; j++) {
+                        double p = propensity_l(L.rx[j], xs, vol, tsv);
+                        if (lane == j) rr = p;
+                        rd += p;
                     }
+                    sr = rd;
+                    if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
+                    if (lead) a.srrate[sv] = sr;
                 }
                 if (sr == 0.0) continue;
                 double rv2 = rand1 * sr;
                 re = 0;
-                cum = rr[0];
-                while (re < R && rv2 > cum) { re++; cum += re < R ? rr[re] : 0.0; }
+                cum = rl_d(rr, 0);
+                while (re < R && rv2 > cum) { re++; cum += re < R ? rl_d(rr, re) : 0.0; }
                 if (re >= R) { err = SSASDPD_ERR_PROPENSITY_OVERFLOW; break; }
             }
-            for (int k = a.jcN[re]; k < a.jcN[re + 1]; k++) {
-                int sp = a.irN[k], d = a.prN[k];
-                xs[sp] += (uint32_t)d;
-                sd += Dd[sp] * d;
+            for (int k = L.jcN[re]; k < L.jcN[re + 1]; k++) {
+                const int sp = L.irN[k], d = L.prN[k];
+                if (lane == sp) xs += (uint32_t)d;
+                sd += rl_d(Dd, sp) * d;
             }
             double rd = 0.0;
-            for (int k = a.jcG[S + re]; k < a.jcG[S + re + 1]; k++) {
-                int j = a.irG[k];
-                double old = rr[j];
-                rr[j] = propensity(a.rx[j], xs, vol, tsv);
-                rd += rr[j] - old;
+            for (int k = L.jcG[S + re]; k < L.jcG[S + re + 1]; k++) {
+                const int j = L.irG[k];
+                const double old = rl_d(rr, j);
+                const double p = propensity_l(L.rx[j], xs, vol, tsv);
+                if (lane == j) rr = p;
+                rd += p - old;
             }
             sr += rd;
-            if (lead) {
-                for (int s = 0; s < S; s++) a.xx[(size_t)sv * S + s] = xs[s];
-                for (int j = 0; j < R; j++) a.rrate[(size_t)sv * R + j] = rr[j];
-                a.srrate[sv] = sr;
-                a.sdrate[sv] = sd;
-            }
+            if (s_lane) a.xx[(size_t)sv * S + lane] = xs;
+            if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
+            if (lead) { a.srrate[sv] = sr; a.sdrate[sv] = sd; }
             n_react++;
             which = re;
+            // ---- queue update (NRMConstant_v5::update, NRMConstant_v5.cpp:89-132)
+            const double tot = sr + sd;
+            double ntv;
+            if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
+            else ntv = rng.exponential() / tot + tt;
+            sib.set(sv, ntv, lane);
+            tree_update_rescan<PL>(a, L, sib, sv, ntv, lane);
         } else {
             // ---- diffusion (simulate_rdme.cpp:312-445)
             event = 1;
             double drand = textbook ? (rand1 * totrate - sr) : rand1 * sd;
             int spec = 0;
-            double cum = S > 0 ? Dd[0] * xs[0] : 0.0;
-            while (spec < S && drand > cum) { spec++; cum += spec < S ? Dd[spec] * xs[spec] : 0.0; }
+            double cum = S > 0 ? rl_d(Dd, 0) * rl_u(xs, 0) : 0.0;
+            while (spec < S && drand > cum) { spec++; cum += spec < S ? rl_d(Dd, spec) * rl_u(xs, spec) : 0.0; }
             if (spec >= S) {
                 if (cum != sd) {
                     double rd = 0.0;
-                    for (int j = 0; j < S; j++) rd += Dd[j] * xs[j];
+                    for (int j = 0; j < S; j++) rd += rl_d(Dd, j) * rl_u(xs, j);
                     sd = rd;
                     if (lead) a.sdrate[sv] = sd;
                 }
                 if (sd == 0.0) continue;
                 drand = cum * rand1;
                 spec = 0;
-                cum = Dd[0] * xs[0];
-                while (spec < S && drand > cum) { spec++; cum += spec < S ? Dd[spec] * xs[spec] : 0.0; }
+                cum = rl_d(Dd, 0) * rl_u(xs, 0);
+                while (spec < S && drand > cum) { spec++; cum += spec < S ? rl_d(Dd, spec) * rl_u(xs, spec) : 0.0; }
                 if (spec >= S) {
                     spec--;
                     bool bad = false;
-                    while (xs[spec] <= 0) {
+                    while (rl_u(xs, spec) <= 0) {
                         spec--;
                         if (spec <= 0) { bad = true; break; }
                     }
@@ -462,113 +684,134 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
             }
             const double r2 = rng.uniform();
             if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
-            double rand2 = r2 * Dd[spec];
+            if (dg > WAVE) { err = SSASDPD_ERR_CAPACITY; break; }
+            // ---- round trip 2: every neighbour's state (lane k fetches
+            // neighbour k into LDS) issued before the walk picks one
+            const bool have = lane < dg && nb_j >= 0;
+            double n_sr = 0.0, n_sd = 0.0, n_Dd = 0.0, n_qt = inf;
+            int n_ty = 1;
+            if (have) {
+                const size_t j = (size_t)nb_j;
+#pragma unroll
+                for (int s2 = 0; s2 < MS; s2++)
+                    if (s2 < S) nbx[lane * MS + s2] = a.xx[j * S + s2];
+#pragma unroll
+                for (int r = 0; r < MR; r++)
+                    if (r < R) nbr[lane * MR + r] = a.rrate[j * R + r];
+                n_sr = a.srrate[j];
+                n_sd = a.sdrate[j];
+                n_Dd = a.Ddiag[j * S + spec];
+                n_qt = a.qtime[j];
+                n_ty = a.type[j];
+            }
+            double rand2 = r2 * rl_d(Dd, spec);
             // destination: walk the neighbour list in order
-            const int dg = a.deg[sv];
-            bool got = false;
-            for (int pass = 0; pass < 2 && !got; pass++) {
+            const double c = have ? nb_D * L.Dmat[spec * T + (nb_t - 1)] : 0.0;
+            int kk = -1;
+            const int m = dg < WAVE ? dg : WAVE;
+            for (int pass = 0; pass < 2 && kk < 0; pass++) {
                 double cum2 = 0.0;
-                for (int base = 0; base < dg && !got; base += WAVE) {
-                    int k = base + lane;
-                    double c = 0.0;
-                    int jj = -1;
-                    if (k < dg) {
-                        size_t e = (size_t)k * n + sv;
-                        jj = a.nidx[e];
-                        int tj = T == 1 ? 1 : a.type[jj];
-                        c = a.Dij[e] * a.Dmat[(size_t)spec * T + (tj - 1)];
-                    }
-                    int m = min(WAVE, dg - base);
-                    for (int kk = 0; kk < m; kk++) {
-                        cum2 += __shfl(c, kk, WAVE);
-                        if (cum2 > rand2) { dest = __shfl(jj, kk, WAVE); got = true; break; }
-                    }
+                for (int k = 0; k < m; k++) {
+                    cum2 += rl_d(c, k);
+                    if (cum2 > rand2) { kk = k; break; }
                 }
-                if (!got) {
+                if (kk < 0) {
                     if (dest >= 0) break;   // stale destination of an earlier event (simulate_rdme.cpp:368)
                     rand2 = r2 * cum2;
                 }
             }
-            if (!got && dest < 0) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
-            // move one molecule
-            xs[spec]--;
-            if (lead) a.xx[(size_t)sv * S + spec] = xs[spec];
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            uint32_t xd[MAXS];
-            double rrd[MAXR];
+            if (kk < 0 && dest < 0) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            // destination state (lane s / lane j layout): from the prefetch,
+            // or loaded when a stale destination is not in this row
+            uint32_t xd;
+            double rrd, srd, sdd, Ddd, qtd;
+            int tds;
+            if (kk >= 0) {
+                dest = __builtin_amdgcn_readlane(nb_j, kk);
+                xd = s_lane ? nbx[kk * MS + lane] : 0u;
+                rrd = r_lane ? nbr[kk * MR + lane] : 0.0;
+                srd = rl_d(n_sr, kk);
+                sdd = rl_d(n_sd, kk);
+                Ddd = rl_d(n_Dd, kk);
+                qtd = rl_d(n_qt, kk);
+                tds = __builtin_amdgcn_readlane(n_ty, kk);
+            } else {
+                const size_t j = (size_t)dest;
+                xd = s_lane ? a.xx[j * S + lane] : 0u;
+                rrd = r_lane ? a.rrate[j * R + lane] : 0.0;
+                srd = a.srrate[j];
+                sdd = a.sdrate[j];
+                Ddd = a.Ddiag[j * S + spec];
+                qtd = a.qtime[j];
+                tds = a.type[j];
+            }
             const bool self = dest == sv;
-            for (int s = 0; s < S; s++) xd[s] = self ? xs[s] : a.xx[(size_t)dest * S + s];
-            xd[spec]++;
-            if (self) xs[spec] = xd[spec];
-            const int tds = a.type[dest];
-            double srd = self ? sr : a.srrate[dest];
-            double sdd;
+            if (self) { xd = xs; rrd = rr; srd = sr; sdd = sd; Ddd = rl_d(Dd, spec); tds = tsv; }
+            // move one molecule
+            if (lane == spec) xs -= 1u;
+            if (self) xd = xs;
+            if (lane == spec) xd += 1u;
+            if (self) xs = xd;
             if (R > 0) {
-                for (int j = 0; j < R; j++) rrd[j] = self ? rr[j] : a.rrate[(size_t)dest * R + j];
                 double rd = 0.0, rrdel = 0.0;
-                for (int k = a.jcG[spec]; k < a.jcG[spec + 1]; k++) {
-                    int j = a.irG[k];
-                    double old = rr[j];
-                    rr[j] = propensity(a.rx[j], xs, vol, tsv);
-                    rd += rr[j] - old;
-                    if (self) rrd[j] = rr[j];
-                    old = rrd[j];
-                    rrd[j] = propensity(a.rx[j], xd, vol, tds);
-                    rrdel += rrd[j] - old;
-                    if (self) rr[j] = rrd[j];
+                for (int k = L.jcG[spec]; k < L.jcG[spec + 1]; k++) {
+                    const int j = L.irG[k];
+                    double old = rl_d(rr, j);
+                    const double p = propensity_l(L.rx[j], xs, vol, tsv);
+                    if (lane == j) rr = p;
+                    rd += p - old;
+                    if (self) rrd = rr;
+                    old = rl_d(rrd, j);
+                    const double pd = propensity_l(L.rx[j], xd, vol, tds);
+                    if (lane == j) rrd = pd;
+                    rrdel += pd - old;
+                    if (self) rr = rrd;
                 }
                 sr += rd;
                 if (self) srd = sr;
                 srd += rrdel;
                 if (self) sr = srd;
             }
-            sd -= Dd[spec];
-            sdd = self ? sd : a.sdrate[dest];
-            sdd += a.Ddiag[(size_t)dest * S + spec];
+            sd -= rl_d(Dd, spec);
+            if (self) sdd = sd;
+            sdd += Ddd;
             if (self) sd = sdd;
-            if (lead) {
-                if (!self) {
-                    for (int j = 0; j < R; j++) a.rrate[(size_t)sv * R + j] = rr[j];
-                    a.srrate[sv] = sr;
-                    a.sdrate[sv] = sd;
-                }
-                a.xx[(size_t)dest * S + spec] = xd[spec];
-                for (int j = 0; j < R; j++) a.rrate[(size_t)dest * R + j] = rrd[j];
-                a.srrate[dest] = srd;
-                a.sdrate[dest] = sdd;
+            if (!self) {
+                if (lane == spec) a.xx[(size_t)sv * S + spec] = xs;
+                if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
+                if (lead) { a.srrate[sv] = sr; a.sdrate[sv] = sd; }
             }
+            if (lane == spec) a.xx[(size_t)dest * S + spec] = xd;
+            if (r_lane) a.rrate[(size_t)dest * R + lane] = rrd;
+            if (lead) { a.srrate[dest] = srd; a.sdrate[dest] = sdd; }
             n_diff++;
             which = spec;
-            dest_sr = srd;
-            dest_sd = sdd;
-        }
-        // ---- queue updates (NRMConstant_v5::update, NRMConstant_v5.cpp:89-132)
-        {
-            double tot = sr + sd;
-            double old = a.qtime[sv];
-            double nt;
-            if (tot <= 0.0) {
-                nt = inf;
-                if (old != inf) q.active--;
+            // ---- queue updates (NRMConstant_v5::update, NRMConstant_v5.cpp:89-132)
+            const double tot = sr + sd;
+            double ntv;
+            if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
+            else ntv = rng.exponential() / tot + tt;
+            sib.set(sv, ntv, lane);
+            tree_update_rescan<PL>(a, L, sib, sv, ntv, lane);
+            const double dest_tot = srd + sdd;
+            const double dold = self ? ntv : qtd;
+            double ntd;
+            if (dest_tot <= 0.0) {
+                ntd = inf;
+                if (dold != inf) q.active--;
             } else {
-                nt = rng.exponential() / tot + tt;
-                if (old == inf) q.active++;
+                ntd = rng.exponential() / dest_tot + tt;
+                if (dold == inf) q.active++;
             }
-            tr.update(sv, nt, lane);
-        }
-        if (event) {
-            double tot = dest_sr + dest_sd;
-            double old = a.qtime[dest];
-            double nt;
-            if (tot <= 0.0) {
-                nt = inf;
-                if (old != inf) q.active--;
+            if (sib.holds(dest)) {
+                sib.set(dest, ntd, lane);
+                tree_update_rescan<PL>(a, L, sib, dest, ntd, lane);
             } else {
-                nt = rng.exponential() / tot + tt;
-                if (old == inf) q.active++;
+                tree_update_point<PL>(a, L, dest, ntd, lane);
             }
-            tr.update(dest, nt, lane);
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
         if (a.trace && lead) {
             unsigned long long ix = q.events_traced;
@@ -580,6 +823,9 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         }
         q.events_traced++;
     }
+    // write the upper levels back for the next launch
+    __syncthreads();
+    for (int i = lane; i < a.tree_nodes; i += WAVE) { a.tmin[i] = L.lt[i]; a.targ[i] = L.lv[i]; }
     q.rng_pos = rng.pos;
     q.reactions += n_react;
     q.diffusions += n_diff;
@@ -587,8 +833,40 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     if (lead) *a.qs = q;
 }
 
+template <int MS, int MR> static void launch_ms(hipStream_t s, const RdmeArgs &a) {
+    switch (a.pl) {
+    case 1: hipLaunchKernelGGL((k_ssa<1, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    case 2: hipLaunchKernelGGL((k_ssa<2, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    case 4: hipLaunchKernelGGL((k_ssa<4, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    case 8: hipLaunchKernelGGL((k_ssa<8, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    case 16: hipLaunchKernelGGL((k_ssa<16, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    default: hipLaunchKernelGGL((k_ssa<32, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    }
+}
+
 void rdme_take_step(hipStream_t s, const RdmeArgs &a) {
-    hipLaunchKernelGGL(k_ssa, dim3(1), dim3(WAVE), 0, s, a);
+    const int m = a.S > a.R ? a.S : a.R;
+    if (m <= 4) launch_ms<4, 4>(s, a);
+    else if (m <= 8) launch_ms<8, 8>(s, a);
+    else launch_ms<16, 16>(s, a);
+}
+
+// LDS bytes of the event-loop kernel for a given model.
+size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG) {
+    auto a16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    size_t off = 0;
+    off = a16(off + sizeof(double) * tree_nodes);
+    off = a16(off + sizeof(int) * tree_nodes);
+    off = a16(off + sizeof(Rxn) * R);
+    off = a16(off + sizeof(double) * S * T);
+    off = a16(off + sizeof(int) * (R + 1));
+    off = a16(off + sizeof(int) * nnzN);
+    off = a16(off + sizeof(int) * nnzN);
+    off = a16(off + sizeof(int) * (S + R + 1));
+    off = a16(off + sizeof(int) * nnzG);
+    off = a16(off + sizeof(uint32_t) * 64 * 16);
+    off = a16(off + sizeof(double) * 64 * 16);
+    return off;
 }
 
 }  // namespace ssa

@@ -1,0 +1,245 @@
+"""Solver drop-in: the same constructor / compile / run interface as
+spatialpy/solvers/solver.py:46-607, running the HIP engine in-process
+through the C ABI instead of generating, compiling and spawning a C++
+program."""
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _abi
+from .core.errors import ModelError, SimulationError
+from .core.model import Parameter, Species
+from .lowering import SystemArrays, K_ZERO, K_FIRST, K_AB, K_2X, output_schedule
+
+
+class Result:
+    """In-memory result with spatialpy.core.result.Result's getters
+    (result.py:270-402, 601-660).  Snapshot k is what the reference writes
+    to output{k}.vtk."""
+
+    def __init__(self, model, data, steps, sa):
+        self.model = model
+        self.data = data
+        self.steps = steps
+        self.sa = sa
+        self.success = True
+        self.timeout = False
+        self.trajectories = [self]
+
+    def get_timespan(self):
+        return self.model.tspan.items
+
+    def _tindex(self, timepoints):
+        idx = np.arange(len(self.get_timespan()))
+        if timepoints is not None:
+            if isinstance(timepoints, float):
+                raise SimulationError("timepoints argument must be an integer, the index of time timespan")
+            idx = np.atleast_1d(idx[timepoints])
+        return idx
+
+    def get_species(self, species, timepoints=None, concentration=False, deterministic=False, debug=False):
+        name = species.name if isinstance(species, Species) else species
+        if name not in self.model.listOfSpecies:
+            raise SimulationError(f"Species '{name}' not found")
+        s = list(self.model.listOfSpecies).index(name)
+        idx = self._tindex(timepoints)
+        if deterministic:
+            out = self.data["C"][idx, :, s].astype(float)
+        else:
+            out = self.data["xx"][idx, :, s].astype(float)
+            if concentration:
+                out = out / (self.sa.mass / self.data["rho"][idx])
+        return out[0] if out.shape[0] == 1 else out
+
+    def get_property(self, property_name, timepoints=None):
+        idx = self._tindex(timepoints)
+        if property_name == "v":
+            out = self.data["v"][idx]
+        elif property_name in ("rho", "bvf_phi", "nu"):
+            out = self.data[property_name][idx]
+        elif property_name == "mass":
+            out = np.broadcast_to(self.sa.mass, (len(idx), self.sa.n))
+        elif property_name == "id":
+            out = np.broadcast_to(self.sa.type, (len(idx), self.sa.n))
+        elif property_name == "type":
+            names = {v: k for k, v in self.model.domain.typeNdxMapping.items()}
+            out = np.broadcast_to(np.array([names[t] for t in self.sa.type], dtype=object), (len(idx), self.sa.n))
+        else:
+            raise SimulationError(f"unknown property {property_name}")
+        return out[0] if out.shape[0] == 1 else out
+
+    def __eq__(self, other):
+        if not isinstance(other, Result):
+            return False
+        return all(np.array_equal(self.data[k], other.data[k]) for k in self.data)
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+    def append(self, other):
+        self.trajectories.append(other)
+
+    def __len__(self):
+        return len(self.trajectories)
+
+    def __getitem__(self, i):
+        return self.trajectories[i]
+
+
+class Solver:
+    """solvers/solver.py:46-607.  `exact_replay=True` feeds the engine the
+    std::mt19937_64(seed) stream (with host-libm exponentials), which makes a
+    run event-for-event identical to the reference engine run with -s seed;
+    otherwise the GPU's own counter-based stream is used."""
+
+    def __init__(self, model, debug_level=0, exact_replay=False, flags=0):
+        if model is None:
+            raise SimulationError("A model is required to run the simulation.")
+        self.model = model
+        self.debug_level = debug_level
+        self.is_compiled = False
+        self.h = None
+        self.exact_replay = exact_replay
+        self.flags = flags
+        self.sa = None
+        self.engine = None
+
+    # -- lowering (solver.py:100-419, core/model.py:187-268)
+    def compile(self, debug=False, profile=False):
+        m = self.model
+        params = m.compile_prep()
+        d = m.domain
+        types = d.typeNdxMapping
+        T = len(types) - 1
+        tp = np.array([types[t] for t in d.type_id], dtype=np.int32)
+        species = list(m.listOfSpecies.values())
+        S = len(species)
+        sidx = {s.name: i for i, s in enumerate(species)}
+
+        def sindex(x):
+            return sidx[x.name if isinstance(x, Species) else x]
+
+        # diffusion matrix [S][T] (solver.py:265-286)
+        D = np.zeros((S, T))
+        for i, sp in enumerate(species):
+            coef = sp.diffusion_coefficient
+            if isinstance(coef, str):
+                coef = params[coef]
+            elif isinstance(coef, Parameter):
+                coef = coef.value
+            for name, j in types.items():
+                if j == 0:
+                    continue
+                if sp.restrict_to is None or name in sp.restrict_to:
+                    D[i, j - 1] = float(coef)
+        rx = []
+        for r in m.listOfReactions.values():
+            if not r.massaction:
+                raise SimulationError(f"Reaction {r.name}: custom propensity functions are not supported "
+                                      "by the GPU engine yet (mass-action reactions only)")
+            k = r.marate
+            if isinstance(k, Parameter):
+                k = k.value if k.value is not None else params[k.name]
+            elif isinstance(k, str):
+                k = params[k] if k in params else float(eval(k, {"__builtins__": {}, "math": math}, dict(params)))
+            k = float(k)
+            reac = {sindex(s): c for s, c in r.reactants.items()}
+            prod = {sindex(s): c for s, c in r.products.items()}
+            keys = list(reac.keys())
+            if not reac:
+                kind, a, b = K_ZERO, 0, 0
+            elif len(reac) == 1 and reac[keys[0]] == 2:
+                kind, a, b = K_2X, keys[0], 0
+            elif len(reac) == 1:
+                kind, a, b = K_FIRST, keys[0], 0
+            else:
+                kind, a, b = K_AB, keys[0], keys[1]
+            mask = 0
+            if r.restrict_to:
+                for t in r.restrict_to:
+                    mask |= 1 << types[f"type_{t}"]
+            rx.append((kind, a, b, k, mask, reac, prod))
+        x = d.vertices
+        if not np.count_nonzero(x[:, 1]):
+            dim = 1
+        elif not np.count_nonzero(x[:, 2]):
+            dim = 2
+        else:
+            dim = 3
+        d.dimensions = dim
+        if self.h is None:
+            self.h = d.find_h()
+        if self.h == 0.0:
+            raise ModelError("h (basis function width) can not be zero.")
+        bcs = []
+        for bc in m.listOfBoundaryConditions:
+            lo = [-math.inf if v is None else float(v) for v in bc.bounds[0::2]]
+            hi = [math.inf if v is None else float(v) for v in bc.bounds[1::2]]
+            tt = -1 if bc.type_id is None else types[bc.type_id]
+            if bc.target in m.listOfSpecies:
+                if not bc.deterministic:
+                    raise SimulationError("BoundaryConditions don't work for stochastic species yet.")
+                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_CHEM, species=sidx[bc.target],
+                                value=[float(bc.value)]))
+            elif bc.target == "v":
+                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_V, value=[float(v) for v in bc.value]))
+            elif bc.target in ("nu", "rho"):
+                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_NU if bc.target == "nu" else _abi.BC_RHO,
+                                value=[float(bc.value)]))
+            else:
+                raise SimulationError(f"Unable handle boundary condition for property '{bc.target}'")
+        ts = m.tspan
+        g = tuple(d.gravity) if d.gravity is not None else (0.0, 0.0, 0.0)
+        sa = SystemArrays(
+            x=x, type=tp, mass=d.mass, rho=d.rho, nu=d.nu, fixed=d.fixed.astype(np.int32), n_types=T,
+            dimension=dim, h=float(self.h), dt=float(ts.timestep_size), nt=int(ts.num_timesteps),
+            output_steps=ts.output_steps, static_domain=int(bool(m.staticDomain)), rho0=float(d.rho0),
+            c0=float(d.c0), P0=float(d.P0), gravity=g,
+            bbox=(d.xlim[0], d.xlim[1], d.ylim[0], d.ylim[1], d.zlim[0], d.zlim[1]),
+            u0=np.ascontiguousarray(m.u0.T).astype(np.uint32) if S else None,
+            diffusion_matrix=D if S else None, reactions=rx, enable_rdme=m.enable_rdme, enable_pde=m.enable_pde,
+            bcs=bcs, flags=self.flags)
+        if S:
+            sa.build_sparse()
+        self.sa = sa.normalized()
+        from .engine import Engine
+        self.engine = Engine(self.sa)
+        self.is_compiled = True
+
+    def run(self, number_of_trajectories=1, seed=None, timeout=None, number_of_threads=None, debug=False,
+            profile=False, verbose=True):
+        """solver.py:510-607: trajectory i uses seed+i."""
+        if not self.is_compiled:
+            self.compile(debug=debug, profile=profile)
+        first = None
+        t0 = time.monotonic()
+        for i in range(number_of_trajectories):
+            s = (seed + i) if seed is not None else int.from_bytes(os.urandom(8), "little")
+            if self.exact_replay:
+                data = self._run_replay(s)
+            else:
+                data = self.engine.run(seed=s, fields=("xx", "C", "x", "v", "rho", "bvf_phi", "nu"))
+            res = Result(self.model, data, data["steps"], self.sa)
+            if timeout is not None and time.monotonic() - t0 > timeout:
+                res.timeout = True
+            if first is None:
+                first = res
+            else:
+                first.append(res)
+            if res.timeout:
+                break
+        return first
+
+    def _run_replay(self, seed):
+        from .engine import mt19937_64_stream
+        n = 1 << 20
+        while True:
+            words, expo = mt19937_64_stream(seed & 0xFFFFFFFFFFFFFFFF, n)
+            try:
+                return self.engine.run(words=words, expo=expo, fields=("xx", "C", "x", "v", "rho", "bvf_phi", "nu"))
+            except SimulationError as e:
+                if e.code != 9 or n >= (1 << 31):
+                    raise
+                n *= 4

@@ -1,0 +1,115 @@
+"""The Solver drop-in, tested like the reference's own integration tests
+(test/integration_tests/test_solver.py:120-200, test/system_tests/
+test_compiler.py:31-36 with test/models/birth_death.py)."""
+import numpy as np
+import pytest
+
+import spatialpy_amd as sp
+from spatialpy_amd import workloads
+
+
+def birth_death_model(seed=1):
+    """test/models/birth_death.py:19-50 written against this package."""
+    model = sp.Model(name="Spatial Birth-Death")
+    domain = sp.Domain.create_2D_domain(xlim=(0, 1), ylim=(0, 1), numx=10, numy=10, type_id="Habitat", fixed=True)
+    model.add_domain(domain)
+    model.add_species(sp.Species(name="Rabbits", diffusion_coefficient=0.1))
+    model.add_initial_condition(sp.ScatterInitialCondition(species="Rabbits", count=100, seed=seed))
+    model.add_parameter([sp.Parameter(name="k_birth", expression=10), sp.Parameter(name="k_death", expression=0.1)])
+    model.add_reaction([sp.Reaction(name="birth", reactants={}, products={"Rabbits": 1}, rate="k_birth"),
+                        sp.Reaction(name="death", reactants={"Rabbits": 1}, products={}, rate="k_death")])
+    model.timespan(sp.TimeSpan.linspace(t=10, num_points=11, timestep_size=1))
+    return model
+
+
+def diffusion_debug_model():
+    """test/integration_tests/test_solver.py:28-46 (50x50 at D=0.01) with 5000
+    molecules at the origin and t=0..1 instead of 1e5 and t=0..10, to keep
+    the event count small."""
+    model = sp.Model("diffusion_debug_test")
+    model.add_species(sp.Species(name="A", diffusion_coefficient=0.01))
+    domain = sp.Domain.create_2D_domain(xlim=[-1, 1], ylim=[-1, 1], numx=50, numy=50, type_id=1, mass=1.0, nu=1.0,
+                                        fixed=True, rho0=1.0, c0=1.0, P0=1.0)
+    model.add_domain(domain)
+    model.add_initial_condition(sp.PlaceInitialCondition("A", 5000, [0, 0, 0]))
+    model.timespan(sp.TimeSpan.linspace(t=1, num_points=11, timestep_size=0.1))
+    return model
+
+
+class _NoEngineSolver(sp.Solver):
+    """Lowering only (the engine needs a GPU)."""
+
+    def compile(self, debug=False, profile=False):
+        try:
+            super().compile(debug, profile)
+        except sp.SimulationError as e:
+            if e.code != 7:      # only "no HIP device" is expected here
+                raise
+
+
+def test_lowering_of_the_reference_birth_death_model():
+    s = _NoEngineSolver(birth_death_model())
+    s.compile()
+    ref = workloads.birth_death_2d()
+    assert np.array_equal(s.sa.x, ref.x) and s.sa.h == ref.h and np.array_equal(s.sa.rho, ref.rho)
+    assert s.sa.dimension == 2 and s.sa.nt == 10 and s.sa.output_steps.tolist() == list(range(11))
+    assert [r[:5] for r in s.sa.reactions] == [(0, 0, 0, 10.0, 0), (1, 0, 0, 0.1, 0)]
+    assert s.sa.u0.sum() == 100
+
+
+def test_propensity_expressions_match_reference_strings():
+    """The golden strings of test/unit_tests/test_reaction.py:47-85, 401-462."""
+    A, B, X = sp.Species("A", 1.0), sp.Species("B", 1.0), sp.Species("X", 1.0)
+    k1 = sp.Parameter("k1", expression=1.0)
+    assert sp.Reaction("r", reactants={A: 1}, products={}, rate=k1).propensity_expression() == "(k1*A)"
+    assert sp.Reaction("r", reactants={X: 1, sp.Species("Y", 1.0): 1}, products={}, rate=k1) \
+        .propensity_expression() == "(((k1*X)*Y)/vol)"
+    assert sp.Reaction("r", reactants={X: 2}, products={}, rate=k1).propensity_expression() == \
+        "(((k1*X)*(X-1))/vol)"
+    assert sp.Reaction("r", reactants={}, products={A: 1}, rate=k1).propensity_expression() == "(k1*vol)"
+
+
+def test_custom_propensity_is_rejected_with_clear_error():
+    m = birth_death_model()
+    m.add_reaction(sp.Reaction("custom", reactants={"Rabbits": 1}, products={}, propensity_function="k_death*Rabbits"))
+    with pytest.raises(sp.SimulationError, match="custom propensity"):
+        _NoEngineSolver(m).compile()
+
+
+@pytest.mark.gpu
+def test_solver_io_t0_equals_u0():
+    model = diffusion_debug_model()
+    result = model.run(seed=3)
+    A = result.get_species("A", 0)
+    assert not (A - model.u0[0]).any()
+
+
+@pytest.mark.gpu
+def test_same_seed_same_result_and_different_seeds_differ():
+    model = diffusion_debug_model()
+    solver = sp.Solver(model)
+    r1 = solver.run(seed=1)
+    r2 = solver.run(seed=1)
+    r3 = solver.run(seed=100)
+    assert r1 == r2
+    assert r1 != r3
+    assert model.run(seed=1) == model.run(seed=1)
+
+
+@pytest.mark.gpu
+def test_exact_replay_solver_matches_oracle_birth_death(oracle):
+    model = birth_death_model()
+    s = sp.Solver(model, exact_replay=True)
+    res = s.run(seed=7)
+    o = oracle.Oracle(s.sa)
+    ref = o.run(seed=7)
+    assert np.array_equal(res.data["xx"], ref["xx"])
+    counts = res.get_species("Rabbits")
+    assert counts.shape == (11, 121) and counts[0].sum() == 100
+
+
+@pytest.mark.gpu
+def test_ensemble_length():
+    model = birth_death_model()
+    res = model.run(number_of_trajectories=3, seed=2)
+    assert len(res) == 3
