@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define SSASDPD_ABI_VERSION 1
+#define SSASDPD_ABI_VERSION 2
 
 /* Status codes.  The reference prints and calls exit(1) for each failure
  * class (particle.cpp:88-126, 189-206; simulate_rdme.cpp:262-294, 322-349,
@@ -201,11 +201,17 @@ int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps);
  * simulate_threads.cpp:171-312). */
 int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *rng, ssasdpd_outputs *out);
 
-/* Many independent trajectories of the same model in one launch per step
- * (the reference runs them one after another, solver.py:547-605).
- * counts: [n_traj][n_snap][n][S]. */
+/* Many independent trajectories of the same model (the reference runs them
+ * one after another, solver.py:547-605, each a separate process).  The
+ * particle state does not depend on the molecule counts, so it is computed
+ * once per step and shared; every trajectory has its own counts, rates,
+ * event queue and seeded GPU stream, and trajectory b is bit-identical to
+ * ssasdpd_run with rng {kind 0, seed seeds[b]}.  The event loops of all
+ * trajectories run in one launch per step, one workgroup each.
+ *   shared: snapshots of the particle fields (its xx is ignored), may be NULL;
+ *   counts: [n_traj][K][n][S] with K from ssasdpd_num_snapshots, may be NULL. */
 int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
-                      uint32_t *counts, int32_t n_snap);
+                      ssasdpd_outputs *shared, uint32_t *counts);
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 

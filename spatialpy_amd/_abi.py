@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 STATUS_NAMES = {
@@ -119,7 +119,7 @@ def _declare(lib):
     lib.ssasdpd_reset.argtypes = [P, C.POINTER(Rng)]
     lib.ssasdpd_advance.argtypes = [P, C.c_uint32]
     lib.ssasdpd_run.argtypes = [P, C.POINTER(Rng), C.POINTER(Outputs)]
-    lib.ssasdpd_run_batch.argtypes = [P, C.c_int32, _u64p, _up, C.c_int32]
+    lib.ssasdpd_run_batch.argtypes = [P, C.c_int32, _u64p, C.POINTER(Outputs), _up]
     lib.ssasdpd_stats_get.argtypes = [P, C.POINTER(Stats)]
     lib.ssasdpd_get_state.argtypes = [P, C.POINTER(State)]
     lib.ssasdpd_get_neighbors.argtypes = [P, _i64p, _ip, _dp, _dp, _dp]

@@ -170,6 +170,8 @@ struct ssasdpd_engine {
     ~ssasdpd_engine() { release(); }
 
     void release() {
+        dfree(b_xx); dfree(b_rr); dfree(b_sr); dfree(b_sd); dfree(b_qt); dfree(b_tmin); dfree(b_targ);
+        dfree(b_qs); dfree(b_seeds);
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
         dfree(rho); dfree(old_rho); dfree(Frho); dfree(mass); dfree(nu); dfree(bvf);
         dfree(type); dfree(solid); dfree(C); dfree(Q);
@@ -286,6 +288,11 @@ struct ssasdpd_engine {
             // levels above the voxels fit the event loop's LDS), 64 per node above
             pl = 1;
             while (pl < 32 && (n + 64 * pl - 1) / (64 * pl) > 3072) pl *= 2;
+            if (const char *e = getenv("SSASDPD_EVENT_TREE_PL")) {
+                int want = atoi(e);
+                if (want == 1 || want == 2 || want == 4 || want == 8 || want == 16 || want == 32)
+                    if ((n + 64 * want - 1) / (64 * want) <= 3072) pl = want;
+            }
             if ((n + 64 * pl - 1) / (64 * pl) > 3072)
                 throw HipError("too many voxels for the event tree (max 6.2M)", hipErrorUnknown, __FILE__, __LINE__);
             lev_n[0] = n;
@@ -494,12 +501,18 @@ struct ssasdpd_engine {
         return 0;
     }
 
-    int do_step() {
-        const unsigned step = cur;
+    // Neighbour search and the three SDPD substeps of one step; the particle
+    // state does not depend on the molecule counts, so a batch of
+    // trajectories shares it.
+    void particle_phase(unsigned step) {
         const bool moving = static_domain == 0;
         if (step == 0 || moving) {
             bin_tree_points();
-            search();                              // substep 0 lists (simulate.cpp:61-63)
+            // substep 0 lists (simulate.cpp:61-63).  On a moving domain the
+            // reference rebuilds them at every step, but after step 0 nothing
+            // reads them before substep 1 rebuilds them again
+            // (simulate.cpp:121-123), so they are built at step 0 only.
+            if (step == 0) search();
         }
         SdpdArgs sa = sdpd_args(step);
         ph_begin(2);
@@ -516,10 +529,26 @@ struct ssasdpd_engine {
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);
         ph_end(2);
-        int rc = 0;
+    }
+
+    void end_step() {
+        ph_collect();
+        stats.ms_neighbors = ph_ms[0] + ph_ms[1];
+        stats.ms_sdpd = ph_ms[2] + ph_ms[5];
+        stats.ms_rates = ph_ms[3];
+        stats.ms_ssa = ph_ms[4];
+        cur++;
+        stats.steps = cur;
+        stats.particle_steps = (uint64_t)cur * n;
+        stats.max_degree = max_deg_seen;
+    }
+
+    int do_step() {
+        const unsigned step = cur;
+        particle_phase(step);
         if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
             RdmeArgs ra = rdme_args(step);
-            if (moving || !rdme_initialized) {
+            if (static_domain == 0 || !rdme_initialized) {
                 ph_begin(3);
                 rdme_rates(st, ra);
                 rdme_queue_build(st, ra, scan_tmp, qwork);
@@ -533,16 +562,126 @@ struct ssasdpd_engine {
             rdme_take_step(st, ra);
             ph_end(4);
         }
-        rc = check(step);
-        ph_collect();
-        stats.ms_neighbors = ph_ms[0] + ph_ms[1];
-        stats.ms_sdpd = ph_ms[2] + ph_ms[5];
-        stats.ms_rates = ph_ms[3];
-        stats.ms_ssa = ph_ms[4];
-        cur++;
-        stats.steps = cur;
-        stats.particle_steps = (uint64_t)cur * n;
-        stats.max_degree = max_deg_seen;
+        int rc = check(step);
+        end_step();
+        return rc;
+    }
+
+    // ---- batched ensembles: B trajectories of the same model, seeded
+    // streams, one event-loop workgroup per trajectory.  Per trajectory:
+    // counts [n][S], rates [n][R], srrate / sdrate / firing times [n], the
+    // event tree and the queue scalars.
+    int bcap = 0;
+    uint32_t *b_xx = nullptr;
+    double *b_rr = nullptr, *b_sr = nullptr, *b_sd = nullptr, *b_qt = nullptr, *b_tmin = nullptr;
+    int *b_targ = nullptr;
+    QueueState *b_qs = nullptr;
+    unsigned long long *b_seeds = nullptr;
+    std::vector<uint64_t> b_seed_h;
+    std::vector<char> b_init;
+
+    void batch_alloc(int B) {
+        if (B <= bcap) return;
+        dfree(b_xx); dfree(b_rr); dfree(b_sr); dfree(b_sd); dfree(b_qt); dfree(b_tmin); dfree(b_targ);
+        dfree(b_qs); dfree(b_seeds);
+        const size_t nn = n;
+        b_xx = dalloc<uint32_t>(nn * S * B);
+        b_rr = dalloc<double>(nn * std::max(R, 1) * B);
+        b_sr = dalloc<double>(nn * B);
+        b_sd = dalloc<double>(nn * B);
+        b_qt = dalloc<double>(nn * B);
+        b_tmin = dalloc<double>((size_t)std::max(tree_nodes, 1) * B);
+        b_targ = dalloc<int>((size_t)std::max(tree_nodes, 1) * B);
+        b_qs = dalloc<QueueState>(B);
+        b_seeds = dalloc<unsigned long long>(B);
+        bcap = B;
+    }
+
+    // The arguments of trajectory b alone (queue build), or of the whole
+    // batch (b < 0: base pointers plus strides for the event loop).
+    RdmeArgs batch_args(unsigned step, int b) {
+        RdmeArgs a = rdme_args(step);
+        const size_t nn = n;
+        a.bs_xx = nn * S;
+        a.bs_rr = nn * std::max(R, 1);
+        a.bs_n = nn;
+        a.bs_tree = (size_t)std::max(tree_nodes, 1);
+        a.rng_kind = 0;
+        a.words = nullptr; a.expo = nullptr; a.n_words = 0;
+        a.trace = nullptr; a.trace_cap = 0;
+        const size_t o = b < 0 ? 0 : (size_t)b;
+        a.xx = b_xx + o * a.bs_xx;
+        a.rrate = b_rr + o * a.bs_rr;
+        a.srrate = b_sr + o * a.bs_n;
+        a.sdrate = b_sd + o * a.bs_n;
+        a.qtime = b_qt + o * a.bs_n;
+        a.tmin = b_tmin + o * a.bs_tree;
+        a.targ = b_targ + o * a.bs_tree;
+        a.qs = b_qs + o;
+        if (b < 0) a.seeds = b_seeds;
+        else a.seed = b_seed_h[b];
+        return a;
+    }
+
+    void batch_reset(int B, const uint64_t *seeds) {
+        ssasdpd_rng r{};
+        r.kind = 0;
+        r.seed = seeds[0];
+        reset(&r);
+        batch_alloc(B);
+        b_seed_h.assign(seeds, seeds + B);
+        b_init.assign(B, 0);
+        h2d(st, b_seeds, (const unsigned long long *)seeds, B);
+        const size_t nn = n;
+        for (int b = 0; b < B; b++) h2d(st, b_xx + (size_t)b * nn * S, u0.data(), nn * S);
+        HIP_OK(hipMemsetAsync(b_sr, 0, sizeof(double) * nn * B, st));
+        HIP_OK(hipMemsetAsync(b_sd, 0, sizeof(double) * nn * B, st));
+        std::vector<QueueState> q0(B);
+        memset(q0.data(), 0, sizeof(QueueState) * B);
+        h2d(st, b_qs, q0.data(), B);
+        HIP_OK(hipStreamSynchronize(st));
+    }
+
+    int batch_step(int B) {
+        const unsigned step = cur;
+        particle_phase(step);
+        int rc = check(step);
+        if (rc) return rc;
+        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
+            bool any = false;
+            ph_begin(3);
+            for (int b = 0; b < B; b++) {
+                if (static_domain && b_init[b]) continue;
+                RdmeArgs ra = batch_args(step, b);
+                rdme_rates(st, ra);
+                rdme_queue_build(st, ra, scan_tmp, qwork);
+                any = true;
+            }
+            ph_end(3);
+            std::vector<QueueState> q(B);
+            if (any) {
+                d2h(st, q.data(), b_qs, B);
+                HIP_OK(hipStreamSynchronize(st));
+                for (int b = 0; b < B; b++) if (q[b].initialized == 1) b_init[b] = 1;
+            }
+            ph_begin(4);
+            rdme_take_step(st, batch_args(step, -1), B);
+            ph_end(4);
+            d2h(st, q.data(), b_qs, B);
+            HIP_OK(hipStreamSynchronize(st));
+            stats.reactions = stats.diffusions = stats.rng_words_used = 0;
+            for (int b = 0; b < B; b++) {
+                stats.reactions += q[b].reactions;
+                stats.diffusions += q[b].diffusions;
+                stats.rng_words_used += q[b].rng_pos;
+                if (q[b].err && !rc) {
+                    char buf[128];
+                    snprintf(buf, sizeof buf, "SSA failed at step %u in trajectory %d", step, b);
+                    rc = set_error(q[b].err, buf);
+                }
+            }
+        }
+        end_step();
         return rc;
     }
 
@@ -693,9 +832,55 @@ int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
     GUARD_END
 }
 
-int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, uint32_t *counts, int32_t n_snap) {
-    (void)e; (void)n_traj; (void)seeds; (void)counts; (void)n_snap;
-    return set_error(SSASDPD_ERR_UNSUPPORTED, "batched runs are not available in this build");
+int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, ssasdpd_outputs *shared,
+                      uint32_t *counts) {
+    GUARD_BEGIN
+    if (n_traj <= 0 || !seeds) return set_error(SSASDPD_ERR_BAD_DESC, "run_batch: n_traj must be positive and seeds given");
+    if (e->S > 0 && e->lds_bytes == 0) return set_error(SSASDPD_ERR_BAD_DESC, "run_batch: no RDME queue");
+    if (e->trace_cap > 0) return set_error(SSASDPD_ERR_UNSUPPORTED, "run_batch: event trace is single-trajectory only");
+    const int B = n_traj;
+    int n_snap = 0;
+    {
+        uint32_t nx = 0;
+        size_t ix = 0;
+        for (uint32_t step = 0; step < e->nt; step++)
+            if (step >= nx) { n_snap++; nx = ix < e->out_steps.size() ? e->out_steps[ix] : 0xFFFFFFFFu; ix++; }
+        n_snap++;
+    }
+
+    e->batch_reset(B, seeds);
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t per = (size_t)e->n * e->S;
+    auto snap = [&](int k) {
+        if (shared && k < shared->n_snapshots) {
+            uint32_t *keep = shared->xx;
+            shared->xx = nullptr;
+            e->snapshot(shared, k);
+            shared->xx = keep;
+        }
+        if (counts && e->S > 0) {
+            for (int b = 0; b < B; b++)
+                d2h(e->st, counts + ((size_t)b * n_snap + k) * per, e->b_xx + (size_t)b * per, per);
+            HIP_OK(hipStreamSynchronize(e->st));
+        }
+    };
+    uint32_t next = 0;
+    size_t idx = 0;
+    int k = 0;
+    for (uint32_t step = 0; step < e->nt; step++) {
+        if (step >= next) {
+            if (k < n_snap) snap(k);
+            k++;
+            next = idx < e->out_steps.size() ? e->out_steps[idx] : 0xFFFFFFFFu;
+            idx++;
+        }
+        int rc = e->batch_step(B);
+        if (rc) return rc;
+    }
+    if (k < n_snap) snap(k);
+    e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+    GUARD_END
 }
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s) {

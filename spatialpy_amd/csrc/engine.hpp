@@ -180,10 +180,15 @@ struct RdmeArgs {
     unsigned step;
     double *psum_out;
     int *active_out;
+    // batched ensembles: trajectory b = blockIdx.x (rates: blockIdx.y) uses
+    // its own counts / rates / queue at these element strides; neighbour
+    // lists, Ddiag and vol are shared
+    size_t bs_xx, bs_rr, bs_n, bs_tree;
+    const unsigned long long *seeds;   // per-trajectory seeds, or null
 };
-void rdme_rates(hipStream_t s, const RdmeArgs &a);
+void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
-void rdme_take_step(hipStream_t s, const RdmeArgs &a);
+void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void tree_rebuild(hipStream_t s, const RdmeArgs &a);
 size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG);
 

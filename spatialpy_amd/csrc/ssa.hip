@@ -104,18 +104,21 @@ __device__ __forceinline__ double propensity(const Rxn &re, const uint32_t *xv, 
 __global__ void k_rates(RdmeArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    const int b = blockIdx.y;
+    uint32_t *xx = a.xx + b * a.bs_xx;
+    double *rrate = a.rrate + b * a.bs_rr, *srrate = a.srrate + b * a.bs_n, *sdrate = a.sdrate + b * a.bs_n;
     const int S = a.S, R = a.R;
     uint32_t xv[MAXS];
-    for (int s = 0; s < S; s++) xv[s] = a.xx[(size_t)i * S + s];
+    for (int s = 0; s < S; s++) xv[s] = xx[(size_t)i * S + s];
     double sr = 0.0;
     for (int j = 0; j < R; j++) {
         double vol = a.mass[i] / a.rho[i];
         double p = propensity(a.rx[j], xv, vol, a.type[i]);
-        a.rrate[(size_t)i * R + j] = p;
+        rrate[(size_t)i * R + j] = p;
         sr += p;
     }
-    a.srrate[i] = sr;
-    a.vol[i] = a.mass[i] / a.rho[i];
+    srrate[i] = sr;
+    if (b == 0) a.vol[i] = a.mass[i] / a.rho[i];
     double sd = 0.0;
     const int dg = a.deg[i];
     for (int s = 0; s < S; s++) {
@@ -125,13 +128,13 @@ __global__ void k_rates(RdmeArgs a) {
             double dc = a.Dmat[(size_t)s * a.T + (a.type[a.nidx[e]] - 1)];
             dd += dc * a.Dij[e];
         }
-        a.Ddiag[(size_t)i * S + s] = dd;
+        if (b == 0) a.Ddiag[(size_t)i * S + s] = dd;
         sd += dd * xv[s];
     }
-    a.sdrate[i] = sd;
+    sdrate[i] = sd;
 }
-void rdme_rates(hipStream_t s, const RdmeArgs &a) {
-    hipLaunchKernelGGL(k_rates, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj) {
+    hipLaunchKernelGGL(k_rates, dim3((a.n + 255) / 256, n_traj), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -514,6 +517,18 @@ template <int PL, int MS, int MR>
 __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
+    {
+        const size_t b = blockIdx.x;
+        a.xx += b * a.bs_xx;
+        a.rrate += b * a.bs_rr;
+        a.srrate += b * a.bs_n;
+        a.sdrate += b * a.bs_n;
+        a.qtime += b * a.bs_n;
+        a.tmin += b * a.bs_tree;
+        a.targ += b * a.bs_tree;
+        a.qs += b;
+        if (a.seeds) a.seed = a.seeds[b];
+    }
     QueueState q = *a.qs;
     if (q.initialized != 1 || q.err) return;
     const int S = a.S, R = a.R, n = a.n, T = a.T;
@@ -833,22 +848,23 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     if (lead) *a.qs = q;
 }
 
-template <int MS, int MR> static void launch_ms(hipStream_t s, const RdmeArgs &a) {
+template <int MS, int MR> static void launch_ms(hipStream_t s, const RdmeArgs &a, int nt) {
+    const dim3 g(nt);
     switch (a.pl) {
-    case 1: hipLaunchKernelGGL((k_ssa<1, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
-    case 2: hipLaunchKernelGGL((k_ssa<2, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
-    case 4: hipLaunchKernelGGL((k_ssa<4, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
-    case 8: hipLaunchKernelGGL((k_ssa<8, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
-    case 16: hipLaunchKernelGGL((k_ssa<16, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
-    default: hipLaunchKernelGGL((k_ssa<32, MS, MR>), dim3(1), dim3(WAVE), a.lds_bytes, s, a); break;
+    case 1: hipLaunchKernelGGL((k_ssa<1, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 2: hipLaunchKernelGGL((k_ssa<2, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 4: hipLaunchKernelGGL((k_ssa<4, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 8: hipLaunchKernelGGL((k_ssa<8, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 16: hipLaunchKernelGGL((k_ssa<16, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    default: hipLaunchKernelGGL((k_ssa<32, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
     }
 }
 
-void rdme_take_step(hipStream_t s, const RdmeArgs &a) {
+void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj) {
     const int m = a.S > a.R ? a.S : a.R;
-    if (m <= 4) launch_ms<4, 4>(s, a);
-    else if (m <= 8) launch_ms<8, 8>(s, a);
-    else launch_ms<16, 16>(s, a);
+    if (m <= 4) launch_ms<4, 4>(s, a, n_traj);
+    else if (m <= 8) launch_ms<8, 8>(s, a, n_traj);
+    else launch_ms<16, 16>(s, a, n_traj);
 }
 
 // LDS bytes of the event-loop kernel for a given model.

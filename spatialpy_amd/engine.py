@@ -160,6 +160,39 @@ class Engine:
         _abi.check(self.lib, self.lib.ssasdpd_run(self.h, C.byref(r), C.byref(o)), "ssasdpd_run")
         return res
 
+    def run_batch(self, seeds, fields=("xx", "C", "x", "v", "rho")):
+        """Trajectories seeds[0..B) of the model in one batched run (seeded
+        GPU streams).  Returns a dict with "xx" of shape [B][K][n][S] and the
+        shared particle fields of shape [K][...] (the particle state does not
+        depend on the molecule counts)."""
+        sa = self.sa
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64) & np.uint64(0xFFFFFFFFFFFFFFFF))
+        B = seeds.size
+        steps = output_schedule(sa.output_steps, sa.nt)
+        k = steps.size
+        n = sa.n
+        res = {}
+        o = _abi.Outputs()
+        o.n_snapshots = k
+        if "C" in fields:
+            res["C"] = np.zeros((k, n, self.Sc))
+            o.C = _p(res["C"], C.c_double)
+        for f in ("x", "v"):
+            if f in fields:
+                res[f] = np.zeros((k, n, 3))
+                setattr(o, f, _p(res[f], C.c_double))
+        for f in ("rho", "bvf_phi", "nu"):
+            if f in fields:
+                res[f] = np.zeros((k, n))
+                setattr(o, f, _p(res[f], C.c_double))
+        res["steps"] = np.zeros(k, np.uint32)
+        o.steps = _p(res["steps"], C.c_uint32)
+        counts = np.zeros((B, k, n, self.S), np.uint32)
+        _abi.check(self.lib, self.lib.ssasdpd_run_batch(self.h, B, _p(seeds, C.c_uint64), C.byref(o),
+                                                        _p(counts, C.c_uint32)), "ssasdpd_run_batch")
+        res["xx"] = counts
+        return res
+
 
 def fr_search(pts, q, dim, h):
     """Stand-alone GPU fixed-radius search (lists in ANN order)."""

@@ -215,6 +215,25 @@ class Solver:
             self.compile(debug=debug, profile=profile)
         first = None
         t0 = time.monotonic()
+        if number_of_trajectories > 1 and not self.exact_replay and self.sa.n_species > 0:
+            # one batched GPU run: the particle state is shared, every
+            # trajectory has its own counts, queue and stream (seed + i)
+            base = seed if seed is not None else int.from_bytes(os.urandom(8), "little")
+            seeds = [(base + i) & 0xFFFFFFFFFFFFFFFF for i in range(number_of_trajectories)]
+            fields = ("xx", "C", "x", "v", "rho", "bvf_phi", "nu")
+            data = self.engine.run_batch(seeds, fields=fields)
+            for i in range(number_of_trajectories):
+                d = {f: data[f] for f in fields if f != "xx" and f in data}
+                d["xx"] = data["xx"][i]
+                d["steps"] = data["steps"]
+                res = Result(self.model, d, d["steps"], self.sa)
+                if first is None:
+                    first = res
+                else:
+                    first.append(res)
+            if timeout is not None and time.monotonic() - t0 > timeout:
+                first.timeout = True
+            return first
         for i in range(number_of_trajectories):
             s = (seed + i) if seed is not None else int.from_bytes(os.urandom(8), "little")
             if self.exact_replay:

@@ -189,34 +189,49 @@ def dam_break_2d(nx=200, ny=200, spacing=0.0025, box=(2.0, 1.5), n_mol=1000, D=1
 
 def cylinder_3d(n_target=1_000_000, radius=1.0, length=4.0, rho0=1.0, c0=20.0, nu=1e-2,
                 inflow=1.0, nA=None, nB=None, k=1e-2, D=1e-3, dt=2.5e-4, nt=2000,
-                out_every=None, seed=0, lattice_spacing=None):
+                out_every=None, seed=0, lattice_spacing=None, walls=0, h_factor=1.3):
     """Config 5: fluid filling a cylinder (axis x) on a cubic lattice with
     spacing (pi r^2 L / n_target)^(1/3), inflow v=(inflow,0,0) on the slab
-    x < 2*spacing, species A+B->C."""
+    x < 2*spacing, species A+B->C.  walls > 0 adds that many lattice layers
+    of fixed wall particles (type 2) around the fluid, radius in
+    (radius, radius + walls*spacing], so the fluid is a pipe flow rather than
+    a free jet (the SDPD's background-pressure transport term pushes a free
+    surface outward)."""
     vol_total = math.pi * radius * radius * length
     s = lattice_spacing or (vol_total / n_target) ** (1.0 / 3.0)
     nxs = int(math.floor(length / s)) + 1
-    nr = int(math.floor(radius / s))
+    rw = radius + walls * s
+    nr = int(math.floor(rw / s))
     ys = np.arange(-nr, nr + 1) * s
     Y, Z = np.meshgrid(ys, ys, indexing="ij")
-    mask = (Y * Y + Z * Z) <= radius * radius
-    yz = np.stack([Y[mask], Z[mask]], axis=1)
+    r2 = Y * Y + Z * Z
+    fluid = r2 <= radius * radius
+    wall = (~fluid) & (r2 <= rw * rw) if walls > 0 else np.zeros_like(fluid)
+    yz = np.stack([Y[fluid], Z[fluid]], axis=1)
+    yzw = np.stack([Y[wall], Z[wall]], axis=1)
     xs = np.arange(nxs) * s
-    x = np.concatenate([np.column_stack([np.full(yz.shape[0], xv), yz]) for xv in xs])
-    n = x.shape[0]
+    xf = np.concatenate([np.column_stack([np.full(yz.shape[0], xv), yz]) for xv in xs])
+    xw = (np.concatenate([np.column_stack([np.full(yzw.shape[0], xv), yzw]) for xv in xs])
+          if yzw.shape[0] else np.zeros((0, 3)))
+    x = np.concatenate([xf, xw])
+    nf, n = xf.shape[0], xf.shape[0] + xw.shape[0]
+    typ = np.concatenate([np.ones(nf, np.int32), np.full(n - nf, 2, np.int32)])
+    fixed = (typ == 2).astype(np.int32)
     mass = rho0 * s ** 3
-    nA = n if nA is None else nA
-    nB = n if nB is None else nB
+    nA = nf if nA is None else nA
+    nB = nf if nB is None else nB
     rng = np.random.default_rng(seed)
-    u0 = np.stack([scatter(rng, n, nA), scatter(rng, n, nB), np.zeros(n, np.int64)], axis=1)
+    u0 = np.zeros((n, 3), np.int64)
+    u0[:nf] = np.stack([scatter(rng, nf, nA), scatter(rng, nf, nB), np.zeros(nf, np.int64)], axis=1)
     out_every = out_every or nt
-    sa = SystemArrays(x=x, type=np.ones(n, np.int32), mass=np.full(n, mass), rho=np.full(n, rho0),
-                      nu=np.full(n, nu), fixed=np.zeros(n, np.int32), n_types=1, dimension=3,
-                      h=1.3 * s, dt=dt, nt=nt,
+    nty = 2 if walls > 0 else 1
+    sa = SystemArrays(x=x, type=typ, mass=np.full(n, mass), rho=np.full(n, rho0),
+                      nu=np.full(n, nu), fixed=fixed, n_types=nty, dimension=3,
+                      h=h_factor * s, dt=dt, nt=nt,
                       output_steps=np.arange(0, nt + 1, out_every, dtype=np.uint32),
                       static_domain=0, rho0=rho0, c0=c0, P0=rho0 * c0 * c0,
-                      bbox=(0, length, -radius, radius, -radius, radius),
-                      u0=u0.astype(np.uint32), diffusion_matrix=np.array([[D], [D], [D]]),
+                      bbox=(0, length, -rw, rw, -rw, rw),
+                      u0=u0.astype(np.uint32), diffusion_matrix=np.array([[D, 0.0][:nty]] * 3),
                       reactions=[(K_AB, 0, 1, k, 0, {0: 1, 1: 1}, {2: 1})],
                       bcs=[dict(lo=(-math.inf, -math.inf, -math.inf), hi=(2 * s, math.inf, math.inf),
                                 type=1, target=1, value=(inflow, 0.0, 0.0))])
