@@ -87,7 +87,11 @@ enum ssasdpd_flags {
     /* Skip the sSSA part (diffusion + reactions). */
     SSASDPD_FLAG_NO_RDME = 1u << 2,
     /* Record the event trace (see ssasdpd_get_trace). */
-    SSASDPD_FLAG_TRACE = 1u << 3
+    SSASDPD_FLAG_TRACE = 1u << 3,
+    /* Pair-force terms of pairwiseForce (model.cpp:39-191) in FP32 from
+     * FP64-formed dx, dv, summed in FP64; integration stays FP64.  Off: the
+     * reference's FP64 arithmetic, operation for operation. */
+    SSASDPD_FLAG_FP32_FORCES = 1u << 4
 };
 
 typedef struct ssasdpd_desc {
@@ -212,6 +216,14 @@ int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *rng, ssasdpd_outputs *out)
  *   counts: [n_traj][K][n][S] with K from ssasdpd_num_snapshots, may be NULL. */
 int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
                       ssasdpd_outputs *shared, uint32_t *counts);
+
+/* Evaluate the pair-force kernel (pairwiseForce, model.cpp:39-191) once on
+ * the engine's current state and neighbour lists without advancing:
+ * F starts at gravity, Fbp and Frho at zero.  fp32 selects the FP32 pair
+ * terms.  Outputs: F, Fbp [n][3], Frho [n] (any may be NULL).  Needs lists
+ * from at least one step; static domains return gravity only (the reference
+ * computes no momentum there that any later step reads). */
+int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp, double *Frho);
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 

@@ -1177,6 +1177,26 @@ int oracle_get_state(void *p, ssasdpd_state *st) {
     return 0;
 }
 
+/* pairwiseForce (model.cpp:39-191) evaluated once on the current state and
+ * lists, F starting at gravity and Fbp, Frho at zero; the simulation state
+ * is left untouched.  F, Fbp: [n][3]. */
+int oracle_eval_forces(void *p, double *F, double *Fbp, double *Frho) {
+    Sim *s = (Sim *)p;
+    const size_t n = s->n;
+    std::vector<double> F0 = s->F, B0 = s->Fbp, R0 = s->Frho, Q0 = s->Q;
+    for (size_t i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) { s->F[3 * i + k] = s->grav[k]; s->Fbp[3 * i + k] = 0.0; }
+        s->Frho[i] = 0.0;
+    }
+    std::fill(s->Q.begin(), s->Q.end(), 0.0);
+    for (int i = 0; i < s->n; i++) s->pairwise_force(i);
+    if (F) memcpy(F, s->F.data(), 3 * n * 8);
+    if (Fbp) memcpy(Fbp, s->Fbp.data(), 3 * n * 8);
+    if (Frho) memcpy(Frho, s->Frho.data(), n * 8);
+    s->F = F0; s->Fbp = B0; s->Frho = R0; s->Q = Q0;
+    return 0;
+}
+
 int oracle_get_neighbors(void *p, int64_t *row_ptr, int32_t *idx, double *dist, double *dWdr, double *Dij) {
     Sim *s = (Sim *)p;
     int64_t off = 0;

@@ -35,6 +35,7 @@ def lib():
         L.oracle_stats.argtypes = [P, C.POINTER(_abi.Stats)]
         L.oracle_get_state.argtypes = [P, C.POINTER(_abi.State)]
         L.oracle_get_neighbors.argtypes = [P] + [C.POINTER(C.c_int64), C.POINTER(C.c_int32)] + [C.POINTER(C.c_double)] * 3
+        L.oracle_eval_forces.argtypes = [P] + [C.POINTER(C.c_double)] * 3
         L.oracle_set_trace_capacity.argtypes = [P, C.c_uint64]
         L.oracle_get_trace.argtypes = [P, C.POINTER(_abi.Event), C.c_uint64, C.POINTER(C.c_uint64)]
         sig = [C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double,
@@ -171,6 +172,15 @@ class Oracle:
         L.oracle_get_neighbors(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
                                idx.ctypes.data_as(C.POINTER(C.c_int32)), dp(dist), dp(dwdr), dp(dij))
         return rp, idx, dist, dwdr, dij
+
+    def eval_forces(self):
+        """pairwiseForce once on the current state: (F [n,3], Fbp [n,3], Frho [n])."""
+        n = self.sa.n
+        F, Fbp, Frho = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = lib().oracle_eval_forces(self.h, dp(F), dp(Fbp), dp(Frho))
+        assert rc == 0
+        return F, Fbp, Frho
 
     def run(self, seed=None, words=None, expo=None):
         """Full trajectory with the reference's snapshot schedule."""

@@ -21,6 +21,7 @@ FLAG_TEXTBOOK_SELECTION = 1 << 0
 FLAG_ANN_ORDER_MOVING = 1 << 1
 FLAG_NO_RDME = 1 << 2
 FLAG_TRACE = 1 << 3
+FLAG_FP32_FORCES = 1 << 4
 
 BC_CHEM, BC_V, BC_NU, BC_RHO = 0, 1, 2, 3
 
@@ -129,10 +130,12 @@ def _declare(lib):
     lib.ssasdpd_get_trace.argtypes = [P, C.POINTER(Event), C.c_uint64, _u64p]
     lib.ssasdpd_mt19937_64_stream.argtypes = [C.c_uint64, C.c_uint64, _u64p, _dp]
     lib.ssasdpd_phase_times.argtypes = [P, _dp, _u64p, C.c_int32]
+    lib.ssasdpd_eval_forces.argtypes = [P, C.c_int32, _dp, _dp, _dp]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
                  "ssasdpd_run", "ssasdpd_run_batch", "ssasdpd_stats_get", "ssasdpd_get_state",
                  "ssasdpd_get_neighbors", "ssasdpd_fr_search", "ssasdpd_set_trace_capacity",
-                 "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream", "ssasdpd_phase_times"]:
+                 "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream", "ssasdpd_phase_times",
+                 "ssasdpd_eval_forces"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

@@ -450,6 +450,7 @@ struct ssasdpd_engine {
         a.x = x; a.v = v; a.vt = vt; a.F = F; a.Fbp = Fbp; a.rho = rho; a.old_rho = old_rho; a.Frho = Frho;
         a.bvf = bvf; a.normal = normal; a.mass = mass; a.nu = nu; a.type = type; a.solid = solid; a.C = C; a.Q = Q;
         a.Dmat = Dmat; a.Ndense = Ndense; a.rx = rx; a.bc = bc; a.deg = deg; a.nidx = nidx; a.nr = nr; a.err = derr;
+        a.fp32 = (flags & SSASDPD_FLAG_FP32_FORCES) ? 1 : 0;
         return a;
     }
 
@@ -879,6 +880,34 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
     }
     if (k < n_snap) snap(k);
     e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp, double *Frho) {
+    GUARD_BEGIN
+    if (!e->lists_built) return set_error(SSASDPD_ERR_BAD_DESC, "eval_forces: no neighbour lists yet (advance first)");
+    const size_t n = e->n, n3 = 3 * n;
+    double *dF = dalloc<double>(n3), *dB = dalloc<double>(n3), *dR = dalloc<double>(n);
+    double *dQ = e->Sc > 0 ? dalloc<double>(n * e->Sc) : nullptr;
+    std::vector<double> g0(n3);
+    for (int k = 0; k < 3; k++) std::fill(g0.begin() + k * n, g0.begin() + (k + 1) * n, e->g[k]);
+    h2d(e->st, dF, g0.data(), n3);
+    HIP_OK(hipMemsetAsync(dB, 0, sizeof(double) * n3, e->st));
+    HIP_OK(hipMemsetAsync(dR, 0, sizeof(double) * n, e->st));
+    if (dQ) HIP_OK(hipMemsetAsync(dQ, 0, sizeof(double) * n * e->Sc, e->st));
+    SdpdArgs a = e->sdpd_args(e->cur);
+    a.F = dF; a.Fbp = dB; a.Frho = dR; a.Q = dQ ? dQ : a.Q;
+    a.Sc = dQ ? a.Sc : 0;
+    a.Rc = dQ ? a.Rc : 0;
+    a.fp32 = fp32 ? 1 : 0;
+    sdpd_forces(e->st, a);
+    std::vector<double> tmp(n3);
+    if (F) { d2h(e->st, tmp.data(), dF, n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, F, (int)n); }
+    if (Fbp) { d2h(e->st, tmp.data(), dB, n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, Fbp, (int)n); }
+    if (Frho) d2h(e->st, Frho, dR, n);
+    HIP_OK(hipStreamSynchronize(e->st));
+    dfree(dF); dfree(dB); dfree(dR); dfree(dQ);
     return 0;
     GUARD_END
 }

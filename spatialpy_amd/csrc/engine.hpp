@@ -116,6 +116,7 @@ void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, c
 // SDPD substeps.
 struct SdpdArgs {
     int n, dim, Sc, Rc, T, static_domain, nbc, W;
+    int fp32;                // pair-force terms in FP32 (FP64 accumulation)
     unsigned step;
     double dt, h, rho0, c0, P0, alpha;
     double g[3];

@@ -90,18 +90,26 @@ __device__ __forceinline__ double chem_rhs(const SdpdArgs &a, int i, int r) {
     }
 }
 
-// pairwiseForce (model.cpp:39-191) in FP64, list order.  For static domains
-// only the species flux Q is observable, so the momentum part is skipped.
+// pairwiseForce (model.cpp:39-191), list order.  Real = double is the
+// reference's arithmetic operation for operation; Real = float evaluates the
+// pair terms (kernel gradient, pressure, viscosity, background pressure,
+// transport tensor, continuity) in FP32 from FP64-formed differences dx, dv
+// and sums them into FP64 accumulators.  The species flux Q stays FP64.  For
+// static domains only Q is observable, so the momentum part is skipped.
+template <typename Real>
 __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const size_t n = a.n;
-    const double h = a.h, P0 = a.P0, rho0 = a.rho0;
+    const double h = a.h;
+    const Real hR = (Real)a.h, P0 = (Real)a.P0, rho0 = (Real)a.rho0, alpha = (Real)a.alpha;
     const double xi[3] = {a.x[i], a.x[n + i], a.x[2 * n + i]};
     const double vi[3] = {a.v[i], a.v[n + i], a.v[2 * n + i]};
     const double vti[3] = {a.vt[i], a.vt[n + i], a.vt[2 * n + i]};
-    const double rhoi = a.rho[i], mi = a.mass[i], nui = a.nu[i];
-    const double Pi = P0 * (rhoi / rho0 - 1.0);
+    const Real viR[3] = {(Real)vi[0], (Real)vi[1], (Real)vi[2]};
+    const Real dvti[3] = {(Real)(vi[0] - vti[0]), (Real)(vi[1] - vti[1]), (Real)(vi[2] - vti[2])};
+    const Real rhoi = (Real)a.rho[i], mi = (Real)a.mass[i], nui = (Real)a.nu[i];
+    const Real Pi = P0 * (rhoi / rho0 - (Real)1.0);
     double F[3] = {a.F[i], a.F[n + i], a.F[2 * n + i]};
     double Fbp[3] = {a.Fbp[i], a.Fbp[n + i], a.Fbp[2 * n + i]};
     double Frho = a.Frho[i];
@@ -112,61 +120,68 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     const int ti = a.type[i];
     const bool momentum = a.static_domain == 0;
     const int dg = a.deg[i];
-    double dx[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+    Real dx[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
     for (int k = 0; k < dg; k++) {
         const size_t e = (size_t)k * n + i;
         const int j = a.nidx[e];
-        const double r = a.nr[e];
-        const double R = r / h;
-        if (R > 1.0) continue;
-        if (r == 0.0) continue;
-        const double dWdr = a.alpha * (-12 * r / (h * h)) * ((1 - R) * (1 - R));
-        const double rhoj = a.rho[j], mj = a.mass[j];
-        const double inv_rh = 1.0 / (r + 0.001 * h);
+        const double rd = a.nr[e];
+        if (rd / h > 1.0) continue;
+        if (rd == 0.0) continue;
+        const Real r = (Real)rd;
+        const Real R = r / hR;
+        const Real dWdr = alpha * ((Real)-12 * r / (hR * hR)) * (((Real)1 - R) * ((Real)1 - R));
+        const Real rhoj = (Real)a.rho[j], mj = (Real)a.mass[j];
+        const Real inv_rh = (Real)1.0 / (r + (Real)0.001 * hR);
         if (momentum) {
-            const double xj[3] = {a.x[j], a.x[n + j], a.x[2 * n + j]};
             const double vj[3] = {a.v[j], a.v[n + j], a.v[2 * n + j]};
             const double vtj[3] = {a.vt[j], a.vt[n + j], a.vt[2 * n + j]};
-            const double nuj = a.nu[j];
-            double dv_dx = 0.0;
+            const Real nuj = (Real)a.nu[j];
+            Real dv_dx = 0;
             for (int d = 0; d < a.dim; d++) {
-                dx[d] = xi[d] - xj[d];
-                dv[d] = vi[d] - vj[d];
+                dx[d] = (Real)(xi[d] - a.x[d * n + j]);
+                dv[d] = (Real)(vi[d] - vj[d]);
                 dv_dx += dv[d] * dx[d];
             }
-            const double Pj = P0 * (rhoj / rho0 - 1.0);
-            double pg = Pi / (rhoi * rhoi) + Pj / (rhoj * rhoj);
+            const Real vjR[3] = {(Real)vj[0], (Real)vj[1], (Real)vj[2]};
+            const Real dvtj[3] = {(Real)(vj[0] - vtj[0]), (Real)(vj[1] - vtj[1]), (Real)(vj[2] - vtj[2])};
+            const Real Pj = P0 * (rhoj / rho0 - (Real)1.0);
+            Real pg = Pi / (rhoi * rhoi) + Pj / (rhoj * rhoj);
             if (pg < 0) pg = -Pi / (rhoi * rhoi) + Pj / (rhoj * rhoj);
-            const double fp = -1.0 * mj * pg * dWdr / (r + 0.001 * h);
-            const double fv = mj * (2.0 * (nui * nuj) / (nui + nuj)) * (1 / (r + 0.001 * h)) * dWdr / ((rhoi * rhoj));
-            const double mri = mi / rhoi, mrj = mj / rhoj;
-            const double vsum = mri * mri + mrj * mrj;
-            const double fbp = -10.0 * P0 * (1.0 / mi) * vsum * dWdr / (r + 0.001 * h);
-            double ft[3];
+            const Real fp = (Real)-1.0 * mj * pg * dWdr / (r + (Real)0.001 * hR);
+            const Real fv = mj * ((Real)2.0 * (nui * nuj) / (nui + nuj)) * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr /
+                            ((rhoi * rhoj));
+            const Real mri = mi / rhoi, mrj = mj / rhoj;
+            const Real vsum = mri * mri + mrj * mrj;
+            const Real fbp = (Real)-10.0 * P0 * ((Real)1.0 / mi) * vsum * dWdr / (r + (Real)0.001 * hR);
+            Real ft[3];
             for (int p = 0; p < 3; p++) {
-                double acc = 0.0;
-                double Tp[3];
+                Real acc = 0;
+                Real Tp[3];
                 for (int q = 0; q < 3; q++)
-                    Tp[q] = 0.5 * ((rhoi * vi[p] * (vti[q] - vi[q])) + (rhoj * vj[p] * (vtj[q] - vj[q])));
+                    Tp[q] = (Real)0.5 * ((rhoi * viR[p] * (-dvti[q])) + (rhoj * vjR[p] * (-dvtj[q])));
                 acc = Tp[0] * dx[0] + Tp[1] * dx[1] + Tp[2] * dx[2];
-                ft[p] = (1.0 / mi) * vsum * acc * dWdr / (r + 0.001 * h);
+                ft[p] = ((Real)1.0 / mi) * vsum * acc * dWdr / (r + (Real)0.001 * hR);
             }
             for (int d = 0; d < a.dim; d++) {
-                F[d] += fp * dx[d] + fv * dv[d] + ft[d];
-                Fbp[d] += fbp * dx[d];
+                F[d] += (double)(fp * dx[d] + fv * dv[d] + ft[d]);
+                Fbp[d] += (double)(fbp * dx[d]);
             }
-            Frho = Frho + rhoi * (mj / rhoj) * dv_dx * (1 / (r + 0.001 * h)) * dWdr -
-                   0.0 * h * rhoi * a.c0 * mj * 2.0 * (rhoj / rhoi - 1.0) * (r * r / (r * r + 0.01 * h * h)) *
-                       (1.0 / (r + 0.001 * h)) * dWdr / rhoj -
-                   (mj / rhoj) *
-                       (rhoi * ((vi[0] - vti[0]) * dx[0] + (vi[1] - vti[1]) * dx[1] + (vi[2] - vti[2]) * dx[2]) +
-                        rhoj * ((vj[0] - vtj[0]) * dx[0] + (vj[1] - vtj[1]) * dx[1] + (vj[2] - vtj[2]) * dx[2])) *
-                       inv_rh * dWdr;
+            Frho = Frho + (double)(rhoi * (mj / rhoj) * dv_dx * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr -
+                                   (Real)0.0 * hR * rhoi * (Real)a.c0 * mj * (Real)2.0 * (rhoj / rhoi - (Real)1.0) *
+                                       (r * r / (r * r + (Real)0.01 * hR * hR)) * ((Real)1.0 / (r + (Real)0.001 * hR)) *
+                                       dWdr / rhoj -
+                                   (mj / rhoj) *
+                                       (rhoi * (dvti[0] * dx[0] + dvti[1] * dx[1] + dvti[2] * dx[2]) +
+                                        rhoj * (dvtj[0] * dx[0] + dvtj[1] * dx[1] + dvtj[2] * dx[2])) *
+                                       inv_rh * dWdr);
         }
         if (Sc > 0) {
-            const double wfd = (1.0 / (r + 0.001 * h)) * dWdr;
-            const double dQc_base = 2.0 * ((mi * mj) / (mi + mj)) * ((rhoi + rhoj) / (rhoi * rhoj)) * (r * r) * wfd /
-                                    ((r * r) + 0.01 * h * h);
+            const double Rd = rd / h;
+            const double dWdrd = a.alpha * (-12 * rd / (h * h)) * ((1 - Rd) * (1 - Rd));
+            const double rhoid = a.rho[i], rhojd = a.rho[j], mid = a.mass[i], mjd = a.mass[j];
+            const double wfd = (1.0 / (rd + 0.001 * h)) * dWdrd;
+            const double dQc_base = 2.0 * ((mid * mjd) / (mid + mjd)) * ((rhoid + rhojd) / (rhoid * rhojd)) *
+                                    (rd * rd) * wfd / ((rd * rd) + 0.01 * h * h);
             for (int s = 0; s < Sc; s++) {
                 const long long kk = (long long)a.Sc * (ti - 1) + s;   // type-major, model.cpp:163
                 const double D = (kk < (long long)a.Sc * a.T) ? a.Dmat[kk] : 0.0;
@@ -266,7 +281,10 @@ __global__ void k_bvf(SdpdArgs a) {
 static dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void sdpd_predict(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_predict, grid1(a.n), dim3(256), 0, s, a); }
-void sdpd_forces(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_forces, grid1(a.n), dim3(256), 0, s, a); }
+void sdpd_forces(hipStream_t s, const SdpdArgs &a) {
+    if (a.fp32) hipLaunchKernelGGL(k_forces<float>, grid1(a.n), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_forces<double>, grid1(a.n), dim3(256), 0, s, a);
+}
 void sdpd_correct(hipStream_t s, const SdpdArgs &a) {
     if (!a.static_domain) hipLaunchKernelGGL(k_correct, grid1(a.n), dim3(256), 0, s, a);
 }

@@ -131,6 +131,18 @@ class Engine:
             _p(dij, C.c_double)), "nbrs")
         return rp, idx, dist, dwdr, dij
 
+    def eval_forces(self, fp32=False):
+        """The pair-force kernel evaluated once on the current state:
+        (F [n,3], Fbp [n,3], Frho [n])."""
+        n = self.sa.n
+        F = np.zeros((n, 3))
+        Fbp = np.zeros((n, 3))
+        Frho = np.zeros(n)
+        _abi.check(self.lib, self.lib.ssasdpd_eval_forces(self.h, 1 if fp32 else 0, _p(F, C.c_double),
+                                                          _p(Fbp, C.c_double), _p(Frho, C.c_double)),
+                   "ssasdpd_eval_forces")
+        return F, Fbp, Frho
+
     def run(self, seed=None, words=None, expo=None, fields=("xx", "C", "x", "v", "rho")):
         """Full trajectory with the reference's snapshot schedule."""
         sa = self.sa
