@@ -495,7 +495,7 @@ struct ssasdpd_engine {
             stats.rng_words_used = q.rng_pos;
             if (q.err) {
                 char buf[96];
-                snprintf(buf, sizeof buf, "SSA failed at step %u", step);
+                snprintf(buf, sizeof buf, "SSA failed at step %u", q.err_step);
                 return set_error(q.err, buf);
             }
         }
@@ -542,6 +542,46 @@ struct ssasdpd_engine {
         stats.steps = cur;
         stats.particle_steps = (uint64_t)cur * n;
         stats.max_degree = max_deg_seen;
+    }
+
+    // Static domains without deterministic species: after step 0 the
+    // particle phase leaves the state unchanged (no motion, boundary
+    // conditions idempotent, rates and vol built once), so the event loops
+    // of all remaining steps run in one launch with in-kernel snapshots.
+    bool fusable_static() const {
+        return static_domain && Sc == 0 && S > 0 && !(flags & SSASDPD_FLAG_NO_RDME) && cur > 0;
+    }
+    bool fusable() const { return fusable_static() && rdme_initialized; }
+
+    // Steps [cur, s_end) of the event loop in one launch for n_traj
+    // trajectories (ra: single-trajectory or batch arguments).  snap: the
+    // step before which each snapshot is taken (s_end: after the last step);
+    // d_xx / d_tot: device snapshot buffers or null.
+    void ssa_fused(RdmeArgs ra, int n_traj, unsigned s_end, const std::vector<unsigned> &snap, uint32_t *d_xx,
+                   size_t bs_snap, uint32_t *d_tot) {
+        unsigned *d_steps = nullptr;
+        if (!snap.empty()) {
+            d_steps = dalloc<unsigned>(snap.size());
+            h2d(st, d_steps, snap.data(), snap.size());
+        }
+        ra.step = cur;
+        ra.step_end = s_end;
+        ra.t0 = cur * dt;
+        ra.snap_step = d_steps;
+        ra.n_snap = (int)snap.size();
+        ra.snap_xx = d_xx;
+        ra.bs_snap = bs_snap;
+        ra.snap_tot = d_tot;
+        ph_begin(4);
+        rdme_take_step(st, ra, n_traj);
+        ph_end(4);
+        HIP_OK(hipStreamSynchronize(st));
+        dfree(d_steps);
+        ph_collect();
+        stats.ms_ssa = ph_ms[4];
+        cur = s_end;
+        stats.steps = cur;
+        stats.particle_steps = (uint64_t)cur * n;
     }
 
     int do_step() {
@@ -668,21 +708,35 @@ struct ssasdpd_engine {
             ph_begin(4);
             rdme_take_step(st, batch_args(step, -1), B);
             ph_end(4);
-            d2h(st, q.data(), b_qs, B);
-            HIP_OK(hipStreamSynchronize(st));
-            stats.reactions = stats.diffusions = stats.rng_words_used = 0;
-            for (int b = 0; b < B; b++) {
-                stats.reactions += q[b].reactions;
-                stats.diffusions += q[b].diffusions;
-                stats.rng_words_used += q[b].rng_pos;
-                if (q[b].err && !rc) {
-                    char buf[128];
-                    snprintf(buf, sizeof buf, "SSA failed at step %u in trajectory %d", step, b);
-                    rc = set_error(q[b].err, buf);
-                }
-            }
+            rc = batch_check(B, step);
         }
         end_step();
+        return rc;
+    }
+
+    bool batch_all_init() const {
+        for (char c : b_init)
+            if (!c) return false;
+        return !b_init.empty();
+    }
+
+    // Sum the trajectories' counters into stats; first error wins.
+    int batch_check(int B, unsigned step) {
+        std::vector<QueueState> q(B);
+        d2h(st, q.data(), b_qs, B);
+        HIP_OK(hipStreamSynchronize(st));
+        int rc = 0;
+        stats.reactions = stats.diffusions = stats.rng_words_used = 0;
+        for (int b = 0; b < B; b++) {
+            stats.reactions += q[b].reactions;
+            stats.diffusions += q[b].diffusions;
+            stats.rng_words_used += q[b].rng_pos;
+            if (q[b].err && !rc) {
+                char buf[128];
+                snprintf(buf, sizeof buf, "SSA failed at step %u in trajectory %d", q[b].err ? q[b].err_step : step, b);
+                rc = set_error(q[b].err, buf);
+            }
+        }
         return rc;
     }
 
@@ -801,13 +855,40 @@ int ssasdpd_reset(ssasdpd_engine *e, const ssasdpd_rng *r) {
 int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps) {
     GUARD_BEGIN
     auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t k = 0; k < n_steps && e->cur < e->nt; k++) {
+    const uint32_t target = (uint32_t)std::min<uint64_t>((uint64_t)e->cur + n_steps, e->nt);
+    while (e->cur < target) {
+        if (e->fusable()) {
+            e->ssa_fused(e->rdme_args(e->cur), 1, target, {}, nullptr, 0, nullptr);
+            int rc = e->check(target - 1);
+            if (rc) return rc;
+            break;
+        }
         int rc = e->do_step();
         if (rc) return rc;
     }
     e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
     GUARD_END
+}
+
+// The reference's output schedule (run_simulation, simulate_threads.cpp:
+// 238-247, 284-289) continued from step s0 to the end: the snapshot taken
+// before each step that reaches the next output step, plus the final one.
+// Returns the steps; ks receives the snapshot indices.
+static std::vector<unsigned> plan_snapshots(const ssasdpd_engine *e, uint32_t s0, uint32_t &next, size_t &idx, int &k,
+                                            std::vector<int> &ks) {
+    std::vector<unsigned> steps;
+    for (uint32_t step = s0; step < e->nt; step++) {
+        if (step >= next) {
+            steps.push_back(step);
+            ks.push_back(k++);
+            next = idx < e->out_steps.size() ? e->out_steps[idx] : 0xFFFFFFFFu;
+            idx++;
+        }
+    }
+    steps.push_back(e->nt);
+    ks.push_back(k++);
+    return steps;
 }
 
 int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
@@ -818,6 +899,30 @@ int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
     size_t idx = 0;
     int k = 0;
     for (uint32_t step = 0; step < e->nt; step++) {
+        if (e->fusable()) {
+            // all remaining steps in one launch, count snapshots in-kernel
+            std::vector<int> ks;
+            std::vector<unsigned> sst = plan_snapshots(e, step, next, idx, k, ks);
+            const int K = o ? (int)std::count_if(ks.begin(), ks.end(), [&](int q) { return q < o->n_snapshots; }) : 0;
+            sst.resize(K);
+            const size_t per = (size_t)e->n * e->S;
+            uint32_t *d_xx = (K > 0 && o->xx) ? dalloc<uint32_t>(per * K) : nullptr;
+            e->ssa_fused(e->rdme_args(step), 1, e->nt, sst, d_xx, 0, nullptr);
+            int rc = e->check(e->nt - 1);
+            for (int j = 0; j < K && !rc; j++) {
+                if (d_xx) d2h(e->st, o->xx + (size_t)ks[j] * per, d_xx + (size_t)j * per, per);
+                uint32_t *keep = o->xx;
+                o->xx = nullptr;
+                e->snapshot(o, ks[j]);        // particle fields: constant after step 0
+                o->xx = keep;
+                if (o->steps) o->steps[ks[j]] = sst[j];
+            }
+            HIP_OK(hipStreamSynchronize(e->st));
+            dfree(d_xx);
+            if (rc) return rc;
+            e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            return 0;
+        }
         if (step >= next) {
             if (o && k < o->n_snapshots) e->snapshot(o, k);
             k++;
@@ -852,13 +957,17 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
     e->batch_reset(B, seeds);
     auto t0 = std::chrono::steady_clock::now();
     const size_t per = (size_t)e->n * e->S;
-    auto snap = [&](int k) {
+    auto snap_shared = [&](int k, uint32_t step) {
         if (shared && k < shared->n_snapshots) {
             uint32_t *keep = shared->xx;
             shared->xx = nullptr;
             e->snapshot(shared, k);
             shared->xx = keep;
+            if (shared->steps) shared->steps[k] = step;
         }
+    };
+    auto snap = [&](int k) {
+        snap_shared(k, e->cur);
         if (counts && e->S > 0) {
             for (int b = 0; b < B; b++)
                 d2h(e->st, counts + ((size_t)b * n_snap + k) * per, e->b_xx + (size_t)b * per, per);
@@ -869,6 +978,25 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
     size_t idx = 0;
     int k = 0;
     for (uint32_t step = 0; step < e->nt; step++) {
+        if (e->fusable_static() && e->batch_all_init()) {
+            std::vector<int> ks;
+            std::vector<unsigned> sst = plan_snapshots(e, step, next, idx, k, ks);
+            const int K = (int)sst.size();
+            uint32_t *d_xx = (counts && e->S > 0) ? dalloc<uint32_t>(per * K * B) : nullptr;
+            e->ssa_fused(e->batch_args(step, -1), B, e->nt, sst, d_xx, per * K, nullptr);
+            int rc = e->batch_check(B, e->nt - 1);
+            for (int j = 0; j < K && !rc; j++) {
+                snap_shared(ks[j], sst[j]);
+                if (d_xx)
+                    for (int b = 0; b < B; b++)
+                        d2h(e->st, counts + ((size_t)b * n_snap + ks[j]) * per, d_xx + ((size_t)b * K + j) * per, per);
+            }
+            HIP_OK(hipStreamSynchronize(e->st));
+            dfree(d_xx);
+            if (rc) return rc;
+            e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            return 0;
+        }
         if (step >= next) {
             if (k < n_snap) snap(k);
             k++;

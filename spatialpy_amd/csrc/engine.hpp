@@ -64,7 +64,8 @@ struct QueueState {
     unsigned long long reactions, diffusions, events_traced;
     int initialized;
     int err;
-    int pad[2];
+    unsigned err_step;       // step at which err was set
+    int pad;
 };
 
 // Kd-tree of the tree points (ANN sliding-midpoint, bucket 1), flattened.
@@ -186,6 +187,17 @@ struct RdmeArgs {
     // lists, Ddiag and vol are shared
     size_t bs_xx, bs_rr, bs_n, bs_tree;
     const unsigned long long *seeds;   // per-trajectory seeds, or null
+    // persistent mode: steps [step, step_end) in one launch (static domains
+    // whose particle phase is constant after step 0).  Snapshot k of this
+    // launch is taken before step snap_step[k] (after the last step when
+    // snap_step[k] == step_end): counts into snap_xx + k*n*S (+ b*bs_snap),
+    // species totals into snap_tot[(b*n_snap + k)*S + s]; either may be null.
+    unsigned step_end;
+    const unsigned *snap_step;
+    int n_snap;
+    uint32_t *snap_xx;
+    size_t bs_snap;
+    uint32_t *snap_tot;
 };
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);

@@ -570,14 +570,53 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     const bool lead = lane == 0;
     const bool s_lane = lane < S, r_lane = lane < R;
 
-    double tt = a.t0;
-    const double end_time = a.t0 + a.dt;
-    int dest = -1;
     int err = 0;
-    unsigned long long guard = 0;
     unsigned long long n_react = 0, n_diff = 0;
     const double inf = INFINITY;
 
+    // species totals, lane s holds species s (persistent mode snapshots)
+    uint32_t tot = 0;
+    if (a.snap_tot) {
+        uint32_t part[MS];
+#pragma unroll
+        for (int u = 0; u < MS; u++) part[u] = 0;
+        const size_t ns = (size_t)n * S;
+        for (size_t i = lane; i < ns; i += WAVE) {
+            const int sp = (int)(i % (size_t)S);
+            const uint32_t xv = a.xx[i];
+#pragma unroll
+            for (int u = 0; u < MS; u++)
+                if (u == sp) part[u] += xv;
+        }
+#pragma unroll
+        for (int u = 0; u < MS; u++) {
+            uint32_t v = part[u];
+            for (int o = WAVE / 2; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, WAVE);
+            if (lane == u) tot = v;
+        }
+    }
+    int ks = 0;
+    auto take_snapshot = [&](int k) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (a.snap_xx) {
+            const size_t ns = (size_t)n * S;
+            uint32_t *dst = a.snap_xx + (size_t)blockIdx.x * a.bs_snap + (size_t)k * ns;
+            for (size_t i = lane; i < ns; i += WAVE) dst[i] = a.xx[i];
+        }
+        if (a.snap_tot && s_lane) a.snap_tot[((size_t)blockIdx.x * a.n_snap + k) * S + lane] = tot;
+    };
+
+    const unsigned step_end = a.step_end > a.step ? a.step_end : a.step + 1;
+    for (unsigned stp = a.step; stp < step_end; stp++) {
+    while (ks < a.n_snap && a.snap_step[ks] <= stp) take_snapshot(ks++);
+    // one step of simulate_rdme (simulate_rdme.cpp:53-68): events while the
+    // last one fired at or before the step's end; the destination carried
+    // between diffusion events restarts each step
+    const double t0 = a.step_end > a.step ? (double)stp * a.dt : a.t0;
+    double tt = t0;
+    const double end_time = t0 + a.dt;
+    int dest = -1;
+    unsigned long long guard = 0;
     while (tt <= end_time) {
         if (++guard > a.max_events) { err = SSASDPD_ERR_CAPACITY; break; }
         // ---- selectReaction: global minimum inside the window [L, U]
@@ -644,7 +683,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
             }
             for (int k = L.jcN[re]; k < L.jcN[re + 1]; k++) {
                 const int sp = L.irN[k], d = L.prN[k];
-                if (lane == sp) xs += (uint32_t)d;
+                if (lane == sp) { xs += (uint32_t)d; tot += (uint32_t)d; }
                 sd += rl_d(Dd, sp) * d;
             }
             double rd = 0.0;
@@ -838,6 +877,10 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         }
         q.events_traced++;
     }
+    if (err) { q.err_step = stp; break; }
+    }
+    if (!err)
+        while (ks < a.n_snap) take_snapshot(ks++);
     // write the upper levels back for the next launch
     __syncthreads();
     for (int i = lane; i < a.tree_nodes; i += WAVE) { a.tmin[i] = L.lt[i]; a.targ[i] = L.lv[i]; }
