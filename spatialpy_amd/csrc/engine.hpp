@@ -203,6 +203,9 @@ void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
 void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void tree_rebuild(hipStream_t s, const RdmeArgs &a);
+// one launch over n_models argument blocks (device array), common PL and
+// MS/MR bucket (max over the models of max(S, R)), LDS = max over models
+void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds);
 size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG);
 
 }  // namespace ssa

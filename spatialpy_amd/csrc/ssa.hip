@@ -87,6 +87,59 @@ struct Rng {
     }
 };
 
+// The event loop's stream, drawn 64 words at a time: lane i holds the
+// uniform and the exponential variate of word base + i, computed in
+// parallel (Philox + log for the seeded stream; the supplied words and
+// host-computed exponentials in replay mode); a draw is one v_readlane pair.
+// Every word is consumed either as a uniform or as an exponential, in the
+// reference's draw order, so the values are those of Rng above.
+struct WaveRng {
+    int kind;
+    uint64_t seed, stream;
+    const uint64_t *words;
+    const double *expo;
+    unsigned long long n_words;
+    unsigned long long pos;      // next word
+    unsigned long long base;     // word held by lane 0
+    double u_l, e_l;
+    bool exhausted;
+    __device__ __forceinline__ void fill(int lane) {
+        base = pos & ~63ULL;
+        const unsigned long long k = base + (unsigned long long)lane;
+        if (kind == 1) {
+            const bool ok = k < n_words;
+            u_l = ok ? word_uniform(words[k]) : 0.5;
+            e_l = ok ? expo[k] : 1.0;
+        } else {
+            const uint64_t w = philox64(seed, stream, k);
+            u_l = word_uniform(w);
+            e_l = word_expo(w);
+        }
+    }
+    __device__ __forceinline__ void init(int lane) { fill(lane); }
+    __device__ __forceinline__ int slot(int lane) {
+        if (pos - base >= 64) fill(lane);
+        return (int)(pos - base);
+    }
+    __device__ __forceinline__ double pick(double x, int s) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(x), s);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(x), s);
+        return __hiloint2double(hi, lo);
+    }
+    __device__ __forceinline__ double uniform(int lane) {
+        if (kind == 1 && pos >= n_words) { exhausted = true; return 0.5; }
+        const int s = slot(lane);
+        pos++;
+        return pick(u_l, s);
+    }
+    __device__ __forceinline__ double exponential(int lane) {
+        if (kind == 1 && pos >= n_words) { exhausted = true; return 1.0; }
+        const int s = slot(lane);
+        pos++;
+        return pick(e_l, s);
+    }
+};
+
 // Mass-action propensities (generated functions, solver.py:344-373).
 __device__ __forceinline__ double propensity(const Rxn &re, const uint32_t *xv, double vol, int tp) {
     if (re.mask && !((re.mask >> tp) & 1ULL)) return 0.0;
@@ -366,16 +419,26 @@ __device__ __forceinline__ double rl_d(double x, int lane) {
     int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
     return __hiloint2double(hi, lo);
 }
-// argmin over the wavefront; every lane gets the result (ties -> lower voxel)
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_min(double t) {
+    return fmin(t, dpp_d<CTRL, RMASK>(t));
+}
+// argmin over the wavefront; every lane gets the result.  The minimum time
+// is reduced alone (DPP + v_min_f64); the winner is the lowest lane holding
+// it, and lanes hold increasing voxel ranges, so ties go to the lower voxel.
 __device__ __forceinline__ void wave_argmin_dpp(double &t, int &v) {
-    dpp_step<0xB1, 0xF>(t, v);    // quad_perm [1,0,3,2]
-    dpp_step<0x4E, 0xF>(t, v);    // quad_perm [2,3,0,1]
-    dpp_step<0x141, 0xF>(t, v);   // row_half_mirror
-    dpp_step<0x140, 0xF>(t, v);   // row_mirror
-    dpp_step<0x142, 0xA>(t, v);   // row_bcast15 into rows 1, 3
-    dpp_step<0x143, 0xC>(t, v);   // row_bcast31 into rows 2, 3
-    t = rl_d(t, 63);
-    v = __builtin_amdgcn_readlane(v, 63);
+    double m = t;
+    m = dpp_min<0xB1, 0xF>(m);    // quad_perm [1,0,3,2]
+    m = dpp_min<0x4E, 0xF>(m);    // quad_perm [2,3,0,1]
+    m = dpp_min<0x141, 0xF>(m);   // row_half_mirror
+    m = dpp_min<0x140, 0xF>(m);   // row_mirror
+    m = dpp_min<0x142, 0xA>(m);   // row_bcast15 into rows 1, 3
+    m = dpp_min<0x143, 0xC>(m);   // row_bcast31 into rows 2, 3
+    m = rl_d(m, 63);
+    const unsigned long long hit = __ballot(t == m);
+    const int w = hit ? (int)__builtin_ctzll(hit) : 0;
+    v = __builtin_amdgcn_readlane(v, w);
+    t = m;
 }
 
 // Small per-voxel vectors are held one element per lane ("lane s holds
@@ -469,8 +532,9 @@ __device__ void tree_fix_up(const RdmeArgs &a, Lds &L, int idx, double m, int am
 
 // Voxel v gets time tv; its level-0 node is in sib (already holding tv).
 template <int PL>
-__device__ void tree_update_rescan(const RdmeArgs &a, Lds &L, const Sib<PL> &sib, int v, double tv, int lane) {
-    if (lane == 0) a.qtime[v] = tv;
+__device__ void tree_update_rescan(const RdmeArgs &a, double *qtime, Lds &L, const Sib<PL> &sib, int v, double tv,
+                                   int lane) {
+    if (lane == 0) qtime[v] = tv;
     double m;
     int am;
     sib.argmin(m, am, lane, a.n);
@@ -486,8 +550,8 @@ __device__ void tree_update_rescan(const RdmeArgs &a, Lds &L, const Sib<PL> &sib
 // minimum is updated in place unless v was the node's argmin and got later,
 // in which case the node is reloaded and rescanned.
 template <int PL>
-__device__ void tree_update_point(const RdmeArgs &a, Lds &L, int v, double tv, int lane) {
-    if (lane == 0) a.qtime[v] = tv;
+__device__ void tree_update_point(const RdmeArgs &a, double *qtime, Lds &L, int v, double tv, int lane) {
+    if (lane == 0) qtime[v] = tv;
     const int idx = v / (WAVE * PL);
     const int off = a.lev_off[1] + idx;
     const double om = L.lt[off];
@@ -497,7 +561,7 @@ __device__ void tree_update_point(const RdmeArgs &a, Lds &L, int v, double tv, i
     if (oam == v && tv > om) {
         Sib<PL> sb;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        sb.load(a.qtime, a.n, idx, lane);
+        sb.load(qtime, a.n, idx, lane);
         sb.set(v, tv, lane);
         sb.argmin(m, am, lane, a.n);
     } else if (oam == v) {
@@ -513,23 +577,36 @@ __device__ void tree_update_point(const RdmeArgs &a, Lds &L, int v, double tv, i
     tree_fix_up<PL>(a, L, idx, m, am, om, oam, lane);
 }
 
+// Snapshot k of one trajectory: its counts [n][S] and its species totals
+// (lane s holds total s).
+__device__ __noinline__ void snapshot_counts(const uint32_t *xx, uint32_t *snap_xx, uint32_t *snap_tot, size_t nsS,
+                                             int S, int k, uint32_t tot, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (snap_xx) {
+        uint32_t *dst = snap_xx + (size_t)k * nsS;
+        for (size_t i = lane; i < nsS; i += WAVE) dst[i] = xx[i];
+    }
+    if (snap_tot && lane < S) snap_tot[(size_t)k * S + lane] = tot;
+}
+
+// The event loop of one trajectory on one wavefront.  b: its index in a
+// batch that shares one argument block (strides bs_*), else 0.
 template <int PL, int MS, int MR>
-__global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
+__device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
-    {
-        const size_t b = blockIdx.x;
-        a.xx += b * a.bs_xx;
-        a.rrate += b * a.bs_rr;
-        a.srrate += b * a.bs_n;
-        a.sdrate += b * a.bs_n;
-        a.qtime += b * a.bs_n;
-        a.tmin += b * a.bs_tree;
-        a.targ += b * a.bs_tree;
-        a.qs += b;
-        if (a.seeds) a.seed = a.seeds[b];
-    }
-    QueueState q = *a.qs;
+    // this trajectory's arrays (the argument block itself stays read-only,
+    // so it is never copied to private memory)
+    uint32_t *const xx_ = a.xx + b * a.bs_xx;
+    double *const rrate_ = a.rrate + b * a.bs_rr;
+    double *const srrate_ = a.srrate + b * a.bs_n;
+    double *const sdrate_ = a.sdrate + b * a.bs_n;
+    double *const qtime_ = a.qtime + b * a.bs_n;
+    double *const tmin_ = a.tmin + b * a.bs_tree;
+    int *const targ_ = a.targ + b * a.bs_tree;
+    QueueState *const qs_ = a.qs + b;
+    const unsigned long long seed_ = a.seeds ? a.seeds[b] : a.seed;
+    QueueState q = *qs_;
     if (q.initialized != 1 || q.err) return;
     const int S = a.S, R = a.R, n = a.n, T = a.T;
 
@@ -551,7 +628,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         L.irG = (int *)(smem + off); off = al16(off + sizeof(int) * nnzG);
         nbx = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * WAVE * MS);
         nbr = (double *)(smem + off); off = al16(off + sizeof(double) * WAVE * MR);
-        for (int i = lane; i < a.tree_nodes; i += WAVE) { L.lt[i] = a.tmin[i]; L.lv[i] = a.targ[i]; }
+        for (int i = lane; i < a.tree_nodes; i += WAVE) { L.lt[i] = tmin_[i]; L.lv[i] = targ_[i]; }
         for (int i = lane; i < R; i += WAVE) L.rx[i] = a.rx[i];
         for (int i = lane; i < S * T; i += WAVE) L.Dmat[i] = a.Dmat[i];
         for (int i = lane; i <= R && R > 0; i += WAVE) L.jcN[i] = a.jcN[i];
@@ -563,9 +640,10 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     const int root = a.lev_off[a.nlev];
     const int W = a.W;
 
-    Rng rng;
-    rng.kind = a.rng_kind; rng.seed = a.seed; rng.stream = a.stream_id; rng.words = a.words; rng.expo = a.expo;
+    WaveRng rng;
+    rng.kind = a.rng_kind; rng.seed = seed_; rng.stream = a.stream_id; rng.words = a.words; rng.expo = a.expo;
     rng.n_words = a.n_words; rng.pos = q.rng_pos; rng.exhausted = false;
+    rng.init(lane);
     const bool textbook = a.textbook != 0;
     const bool lead = lane == 0;
     const bool s_lane = lane < S, r_lane = lane < R;
@@ -583,7 +661,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         const size_t ns = (size_t)n * S;
         for (size_t i = lane; i < ns; i += WAVE) {
             const int sp = (int)(i % (size_t)S);
-            const uint32_t xv = a.xx[i];
+            const uint32_t xv = xx_[i];
 #pragma unroll
             for (int u = 0; u < MS; u++)
                 if (u == sp) part[u] += xv;
@@ -596,19 +674,14 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         }
     }
     int ks = 0;
-    auto take_snapshot = [&](int k) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        if (a.snap_xx) {
-            const size_t ns = (size_t)n * S;
-            uint32_t *dst = a.snap_xx + (size_t)blockIdx.x * a.bs_snap + (size_t)k * ns;
-            for (size_t i = lane; i < ns; i += WAVE) dst[i] = a.xx[i];
-        }
-        if (a.snap_tot && s_lane) a.snap_tot[((size_t)blockIdx.x * a.n_snap + k) * S + lane] = tot;
-    };
+    const size_t nsS = (size_t)n * S;
+    uint32_t *const snap_xx = a.snap_xx ? a.snap_xx + b * a.bs_snap : nullptr;
+    uint32_t *const snap_tot = a.snap_tot ? a.snap_tot + b * a.n_snap * S : nullptr;
+    const uint32_t *const xx_all = xx_;
 
     const unsigned step_end = a.step_end > a.step ? a.step_end : a.step + 1;
     for (unsigned stp = a.step; stp < step_end; stp++) {
-    while (ks < a.n_snap && a.snap_step[ks] <= stp) take_snapshot(ks++);
+    while (ks < a.n_snap && a.snap_step[ks] <= stp) snapshot_counts(xx_all, snap_xx, snap_tot, nsS, S, ks++, tot, lane);
     // one step of simulate_rdme (simulate_rdme.cpp:53-68): events while the
     // last one fired at or before the step's end; the destination carried
     // between diffusion events restarts each step
@@ -634,14 +707,14 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
         // ---- round trip 1: the voxel's state, its neighbour row and its
         // level-0 node
         Sib<PL> sib;
-        sib.load(a.qtime, n, sv / (WAVE * PL), lane);
+        sib.load(qtime_, n, sv / (WAVE * PL), lane);
         const double vol = a.vol[sv];
-        double sr = a.srrate[sv], sd = a.sdrate[sv];
+        double sr = srrate_[sv], sd = sdrate_[sv];
         const int tsv = a.type[sv];
         const int dg = a.deg[sv];
-        uint32_t xs = s_lane ? a.xx[(size_t)sv * S + lane] : 0u;       // lane s: count of species s
+        uint32_t xs = s_lane ? xx_[(size_t)sv * S + lane] : 0u;       // lane s: count of species s
         const double Dd = s_lane ? a.Ddiag[(size_t)sv * S + lane] : 0.0;
-        double rr = r_lane ? a.rrate[(size_t)sv * R + lane] : 0.0;     // lane j: rate of reaction j
+        double rr = r_lane ? rrate_[(size_t)sv * R + lane] : 0.0;     // lane j: rate of reaction j
         int nb_j = -1, nb_t = 1;
         double nb_D = 0.0;
         if (lane < W) {
@@ -651,7 +724,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
             if (T > 1) nb_t = a.ntype[e];
         }
         double totrate = sr + sd;
-        const double rand1 = rng.uniform();
+        const double rand1 = rng.uniform(lane);
         if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
         int event;
         int which;
@@ -671,8 +744,8 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                         rd += p;
                     }
                     sr = rd;
-                    if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
-                    if (lead) a.srrate[sv] = sr;
+                    if (r_lane) rrate_[(size_t)sv * R + lane] = rr;
+                    if (lead) srrate_[sv] = sr;
                 }
                 if (sr == 0.0) continue;
                 double rv2 = rand1 * sr;
@@ -695,18 +768,18 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                 rd += p - old;
             }
             sr += rd;
-            if (s_lane) a.xx[(size_t)sv * S + lane] = xs;
-            if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
-            if (lead) { a.srrate[sv] = sr; a.sdrate[sv] = sd; }
+            if (s_lane) xx_[(size_t)sv * S + lane] = xs;
+            if (r_lane) rrate_[(size_t)sv * R + lane] = rr;
+            if (lead) { srrate_[sv] = sr; sdrate_[sv] = sd; }
             n_react++;
             which = re;
             // ---- queue update (NRMConstant_v5::update, NRMConstant_v5.cpp:89-132)
             const double tot = sr + sd;
             double ntv;
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
-            else ntv = rng.exponential() / tot + tt;
+            else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(a, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(a, qtime_, L, sib, sv, ntv, lane);
         } else {
             // ---- diffusion (simulate_rdme.cpp:312-445)
             event = 1;
@@ -719,7 +792,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                     double rd = 0.0;
                     for (int j = 0; j < S; j++) rd += rl_d(Dd, j) * rl_u(xs, j);
                     sd = rd;
-                    if (lead) a.sdrate[sv] = sd;
+                    if (lead) sdrate_[sv] = sd;
                 }
                 if (sd == 0.0) continue;
                 drand = cum * rand1;
@@ -736,7 +809,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                     if (bad) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
                 }
             }
-            const double r2 = rng.uniform();
+            const double r2 = rng.uniform(lane);
             if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
             if (dg > WAVE) { err = SSASDPD_ERR_CAPACITY; break; }
             // ---- round trip 2: every neighbour's state (lane k fetches
@@ -748,14 +821,14 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                 const size_t j = (size_t)nb_j;
 #pragma unroll
                 for (int s2 = 0; s2 < MS; s2++)
-                    if (s2 < S) nbx[lane * MS + s2] = a.xx[j * S + s2];
+                    if (s2 < S) nbx[lane * MS + s2] = xx_[j * S + s2];
 #pragma unroll
                 for (int r = 0; r < MR; r++)
-                    if (r < R) nbr[lane * MR + r] = a.rrate[j * R + r];
-                n_sr = a.srrate[j];
-                n_sd = a.sdrate[j];
+                    if (r < R) nbr[lane * MR + r] = rrate_[j * R + r];
+                n_sr = srrate_[j];
+                n_sd = sdrate_[j];
                 n_Dd = a.Ddiag[j * S + spec];
-                n_qt = a.qtime[j];
+                n_qt = qtime_[j];
                 n_ty = a.type[j];
             }
             double rand2 = r2 * rl_d(Dd, spec);
@@ -792,12 +865,12 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                 tds = __builtin_amdgcn_readlane(n_ty, kk);
             } else {
                 const size_t j = (size_t)dest;
-                xd = s_lane ? a.xx[j * S + lane] : 0u;
-                rrd = r_lane ? a.rrate[j * R + lane] : 0.0;
-                srd = a.srrate[j];
-                sdd = a.sdrate[j];
+                xd = s_lane ? xx_[j * S + lane] : 0u;
+                rrd = r_lane ? rrate_[j * R + lane] : 0.0;
+                srd = srrate_[j];
+                sdd = sdrate_[j];
                 Ddd = a.Ddiag[j * S + spec];
-                qtd = a.qtime[j];
+                qtd = qtime_[j];
                 tds = a.type[j];
             }
             const bool self = dest == sv;
@@ -832,22 +905,22 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
             sdd += Ddd;
             if (self) sd = sdd;
             if (!self) {
-                if (lane == spec) a.xx[(size_t)sv * S + spec] = xs;
-                if (r_lane) a.rrate[(size_t)sv * R + lane] = rr;
-                if (lead) { a.srrate[sv] = sr; a.sdrate[sv] = sd; }
+                if (lane == spec) xx_[(size_t)sv * S + spec] = xs;
+                if (r_lane) rrate_[(size_t)sv * R + lane] = rr;
+                if (lead) { srrate_[sv] = sr; sdrate_[sv] = sd; }
             }
-            if (lane == spec) a.xx[(size_t)dest * S + spec] = xd;
-            if (r_lane) a.rrate[(size_t)dest * R + lane] = rrd;
-            if (lead) { a.srrate[dest] = srd; a.sdrate[dest] = sdd; }
+            if (lane == spec) xx_[(size_t)dest * S + spec] = xd;
+            if (r_lane) rrate_[(size_t)dest * R + lane] = rrd;
+            if (lead) { srrate_[dest] = srd; sdrate_[dest] = sdd; }
             n_diff++;
             which = spec;
             // ---- queue updates (NRMConstant_v5::update, NRMConstant_v5.cpp:89-132)
             const double tot = sr + sd;
             double ntv;
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
-            else ntv = rng.exponential() / tot + tt;
+            else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(a, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(a, qtime_, L, sib, sv, ntv, lane);
             const double dest_tot = srd + sdd;
             const double dold = self ? ntv : qtd;
             double ntd;
@@ -855,14 +928,14 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
                 ntd = inf;
                 if (dold != inf) q.active--;
             } else {
-                ntd = rng.exponential() / dest_tot + tt;
+                ntd = rng.exponential(lane) / dest_tot + tt;
                 if (dold == inf) q.active++;
             }
             if (sib.holds(dest)) {
                 sib.set(dest, ntd, lane);
-                tree_update_rescan<PL>(a, L, sib, dest, ntd, lane);
+                tree_update_rescan<PL>(a, qtime_, L, sib, dest, ntd, lane);
             } else {
-                tree_update_point<PL>(a, L, dest, ntd, lane);
+                tree_update_point<PL>(a, qtime_, L, dest, ntd, lane);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -880,15 +953,46 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
     if (err) { q.err_step = stp; break; }
     }
     if (!err)
-        while (ks < a.n_snap) take_snapshot(ks++);
+        while (ks < a.n_snap) snapshot_counts(xx_all, snap_xx, snap_tot, nsS, S, ks++, tot, lane);
     // write the upper levels back for the next launch
     __syncthreads();
-    for (int i = lane; i < a.tree_nodes; i += WAVE) { a.tmin[i] = L.lt[i]; a.targ[i] = L.lv[i]; }
+    for (int i = lane; i < a.tree_nodes; i += WAVE) { tmin_[i] = L.lt[i]; targ_[i] = L.lv[i]; }
     q.rng_pos = rng.pos;
     q.reactions += n_react;
     q.diffusions += n_diff;
     if (err) q.err = err;
-    if (lead) *a.qs = q;
+    if (lead) *qs_ = q;
+}
+
+template <int PL, int MS, int MR>
+__global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
+    ssa_run<PL, MS, MR>(a, blockIdx.x);
+}
+
+// Ensembles of different models: workgroup g runs the trajectory described
+// by args[g] (its own domain, chemistry, queue and stream).
+template <int PL, int MS, int MR>
+__global__ void __launch_bounds__(WAVE) k_ssa_multi(const RdmeArgs *args) {
+    ssa_run<PL, MS, MR>(args[blockIdx.x], 0);
+}
+
+template <int MS, int MR>
+static void launch_multi_ms(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, size_t lds) {
+    const dim3 g(n_models);
+    switch (pl) {
+    case 1: hipLaunchKernelGGL((k_ssa_multi<1, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    case 2: hipLaunchKernelGGL((k_ssa_multi<2, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    case 4: hipLaunchKernelGGL((k_ssa_multi<4, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    case 8: hipLaunchKernelGGL((k_ssa_multi<8, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    case 16: hipLaunchKernelGGL((k_ssa_multi<16, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    default: hipLaunchKernelGGL((k_ssa_multi<32, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    }
+}
+
+void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds) {
+    if (max_sr <= 4) launch_multi_ms<4, 4>(s, d_args, n_models, pl, lds);
+    else if (max_sr <= 8) launch_multi_ms<8, 8>(s, d_args, n_models, pl, lds);
+    else launch_multi_ms<16, 16>(s, d_args, n_models, pl, lds);
 }
 
 template <int MS, int MR> static void launch_ms(hipStream_t s, const RdmeArgs &a, int nt) {
