@@ -128,10 +128,10 @@ typedef struct ssasdpd_desc {
     const double *diffusion_matrix;   /* [n_species][n_types] species-major     */
     const ssasdpd_reaction *reactions;/* [n_reactions]                          */
     const int32_t *jcN;       /* [R+1] stoichiometry CSC column pointers        */
-    const int32_t *irN;       /* [nnz] species rows                             */
+    const int32_t *irN;       /* [nnz] species rows, ascending per column       */
     const int32_t *prN;       /* [nnz] net change                               */
     const int32_t *jcG;       /* [S+R+1] dependency-graph CSC column pointers   */
-    const int32_t *irG;       /* [nnzG] reaction rows                           */
+    const int32_t *irG;       /* [nnzG] reaction rows, ascending per column     */
     const int32_t *N_dense;   /* [S][R] row-major (chem RHS, model.cpp:183-189) */
 
     const ssasdpd_bc *bcs;    /* [n_bcs]                                        */

@@ -802,6 +802,26 @@ static int validate(const ssasdpd_desc *d) {
         if (q.kind < 0 || q.kind > 3 || q.a < 0 || q.a >= std::max(Smax, 1) || q.b < 0 || q.b >= std::max(Smax, 1))
             return set_error(SSASDPD_ERR_BAD_DESC, "bad reaction table entry");
     }
+    // CSC columns: row indices strictly ascending and in range (the event
+    // loop applies a column as a bitmask in ascending order, as the
+    // reference's loops over these columns do for the matrices model.py builds)
+    auto csc_ok = [](const int32_t *jc, const int32_t *ir, int cols, int rows) {
+        if (jc[0] != 0) return false;
+        for (int c = 0; c < cols; c++) {
+            if (jc[c + 1] < jc[c]) return false;
+            for (int k = jc[c]; k < jc[c + 1]; k++) {
+                if (ir[k] < 0 || ir[k] >= rows) return false;
+                if (k > jc[c] && ir[k] <= ir[k - 1]) return false;
+            }
+        }
+        return true;
+    };
+    if (d->n_species > 0) {
+        if (!csc_ok(d->jcG, d->irG, d->n_species + d->n_reactions, std::max(d->n_reactions, 1)))
+            return set_error(SSASDPD_ERR_BAD_DESC, "dependency graph CSC: rows must be ascending and < n_reactions");
+        if (d->n_reactions > 0 && !csc_ok(d->jcN, d->irN, d->n_reactions, d->n_species))
+            return set_error(SSASDPD_ERR_BAD_DESC, "stoichiometry CSC: rows must be ascending and < n_species");
+    }
     return 0;
 }
 

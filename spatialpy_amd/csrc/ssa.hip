@@ -459,6 +459,44 @@ __device__ __forceinline__ double propensity_l(const Rxn &re, uint32_t x_l, doub
     }
 }
 
+// The model's small tables held one row per lane for the whole run: lane j
+// has reaction j's kind, reactants, rate and type mask; lane c < S+R has
+// the dependency-graph column c as a bitmask of reactions (irG is ascending
+// within a column); lane s has species s's stoichiometric change under each
+// reaction.
+template <int MR> struct LaneModel {
+    int kind, ra, rb;
+    double k;
+    uint64_t mask;
+    uint32_t dep;
+    int dN[MR];
+};
+// Propensity of reaction `lane` from lane-distributed counts: every lane
+// fetches its reactants' counts with one bpermute each.  Same operations as
+// propensity_l, so the same bits.
+template <int MR>
+__device__ __forceinline__ double propensity_par(const LaneModel<MR> &m, uint32_t x_l, double vol, int tp) {
+    const int xa = __shfl((int)x_l, m.ra, WAVE);
+    const int xb = __shfl((int)x_l, m.rb, WAVE);
+    if (m.mask && !((m.mask >> tp) & 1ULL)) return 0.0;
+    switch (m.kind) {
+    case 0: return m.k * vol;
+    case 1: return m.k * xa;
+    case 2: return ((m.k * xa) * xb) / vol;
+    default: return ((m.k * xa) * (xa - 1)) / vol;
+    }
+}
+// Sum over the set bits of `bits` in ascending lane order of lane-held
+// values, added one by one (the reference's sequential order).
+__device__ __forceinline__ double ordered_sum(double acc, double x, uint64_t bits) {
+    while (bits) {
+        const int j = (int)__builtin_ctzll(bits);
+        bits &= bits - 1;
+        acc += rl_d(x, j);
+    }
+    return acc;
+}
+
 // Level-0 children of one level-1 node held in registers: lane l owns the
 // voxels base + l*PL .. base + l*PL + PL-1.
 template <int PL> struct Sib {
@@ -466,11 +504,16 @@ template <int PL> struct Sib {
     int base;
     __device__ __forceinline__ void load(const double *qt, int n, int node, int lane) {
         base = node * WAVE * PL;
+        // unconditional loads (clamped index) keep the round trip in one
+        // basic block; children past n read as +inf
+        double raw[PL];
 #pragma unroll
         for (int p = 0; p < PL; p++) {
-            int c = base + lane * PL + p;
-            t[p] = c < n ? qt[c] : INFINITY;
+            const int c = base + lane * PL + p;
+            raw[p] = qt[c < n ? c : n - 1];
         }
+#pragma unroll
+        for (int p = 0; p < PL; p++) t[p] = base + lane * PL + p < n ? raw[p] : INFINITY;
     }
     __device__ __forceinline__ bool holds(int v) const { return v >= base && v < base + WAVE * PL; }
     __device__ __forceinline__ void set(int v, double tv, int lane) {
@@ -491,14 +534,24 @@ template <int PL> struct Sib {
     }
 };
 
+// The event tree's shape for the loop: lane k holds the offset (lo_l) and
+// node count (ln_l) of level k, read with v_readlane (no scalar registers
+// held for the tables).
+struct TreeShape {
+    int n, nlev;
+    int lo_l, ln_l;
+    __device__ __forceinline__ int off(int k) const { return __builtin_amdgcn_readlane(lo_l, k); }
+    __device__ __forceinline__ int cnt(int k) const { return __builtin_amdgcn_readlane(ln_l, k); }
+};
+
 // Repair the levels above a changed level-1 node value (idx -> (m, am)),
 // rescanning a node only when its stored minimum may have increased.
 template <int PL>
-__device__ void tree_fix_up(const RdmeArgs &a, Lds &L, int idx, double m, int am, double old_m, int old_am,
+__device__ void tree_fix_up(const TreeShape &a, Lds &L, int idx, double m, int am, double old_m, int old_am,
                             int lane) {
     for (int k = 1; k < a.nlev; k++) {
         const int p = idx / WAVE;
-        const int poff = a.lev_off[k + 1] + p;
+        const int poff = a.off(k + 1) + p;
         const double pm = L.lt[poff];
         const int pam = L.lv[poff];
         double nm;
@@ -508,9 +561,9 @@ __device__ void tree_fix_up(const RdmeArgs &a, Lds &L, int idx, double m, int am
             int c = p * WAVE + lane;
             double tt = INFINITY;
             int vv = INT_MAX;
-            if (c < a.lev_n[k]) {
+            if (c < a.cnt(k)) {
                 if (c == idx) { tt = m; vv = am; }
-                else { tt = L.lt[a.lev_off[k] + c]; vv = L.lv[a.lev_off[k] + c]; }
+                else { tt = L.lt[a.off(k) + c]; vv = L.lv[a.off(k) + c]; }
             }
             wave_argmin_dpp(tt, vv);
             nm = tt;
@@ -532,14 +585,14 @@ __device__ void tree_fix_up(const RdmeArgs &a, Lds &L, int idx, double m, int am
 
 // Voxel v gets time tv; its level-0 node is in sib (already holding tv).
 template <int PL>
-__device__ void tree_update_rescan(const RdmeArgs &a, double *qtime, Lds &L, const Sib<PL> &sib, int v, double tv,
+__device__ void tree_update_rescan(const TreeShape &a, double *qtime, Lds &L, const Sib<PL> &sib, int v, double tv,
                                    int lane) {
     if (lane == 0) qtime[v] = tv;
     double m;
     int am;
     sib.argmin(m, am, lane, a.n);
     const int idx = v / (WAVE * PL);
-    const int off = a.lev_off[1] + idx;
+    const int off = a.off(1) + idx;
     const double om = L.lt[off];
     const int oam = L.lv[off];
     if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
@@ -550,10 +603,10 @@ __device__ void tree_update_rescan(const RdmeArgs &a, double *qtime, Lds &L, con
 // minimum is updated in place unless v was the node's argmin and got later,
 // in which case the node is reloaded and rescanned.
 template <int PL>
-__device__ void tree_update_point(const RdmeArgs &a, double *qtime, Lds &L, int v, double tv, int lane) {
+__device__ void tree_update_point(const TreeShape &a, double *qtime, Lds &L, int v, double tv, int lane) {
     if (lane == 0) qtime[v] = tv;
     const int idx = v / (WAVE * PL);
-    const int off = a.lev_off[1] + idx;
+    const int off = a.off(1) + idx;
     const double om = L.lt[off];
     const int oam = L.lv[off];
     double m;
@@ -639,6 +692,32 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     }
     const int root = a.lev_off[a.nlev];
     const int W = a.W;
+    TreeShape tsh;
+    tsh.n = n;
+    tsh.nlev = a.nlev;
+    tsh.lo_l = 0;
+    tsh.ln_l = 0;
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+        if (lane == k) { tsh.lo_l = a.lev_off[k]; tsh.ln_l = a.lev_n[k]; }
+    LaneModel<MR> lm;
+    {
+        lm.kind = 0; lm.ra = 0; lm.rb = 0; lm.k = 0.0; lm.mask = 0; lm.dep = 0;
+        if (lane < R) {
+            const Rxn r = L.rx[lane];
+            lm.kind = r.kind; lm.ra = r.a; lm.rb = r.b; lm.k = r.k; lm.mask = r.mask;
+        }
+        if (lane < S + R)
+            for (int k = L.jcG[lane]; k < L.jcG[lane + 1]; k++) lm.dep |= 1u << L.irG[k];
+#pragma unroll
+        for (int r = 0; r < MR; r++) {
+            int d = 0;
+            if (r < R)
+                for (int k = L.jcN[r]; k < L.jcN[r + 1]; k++)
+                    if (L.irN[k] == lane) d = L.prN[k];
+            lm.dN[r] = lane < S ? d : 0;
+        }
+    }
 
     WaveRng rng;
     rng.kind = a.rng_kind; rng.seed = seed_; rng.stream = a.stream_id; rng.words = a.words; rng.expo = a.expo;
@@ -711,18 +790,24 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         const double vol = a.vol[sv];
         double sr = srrate_[sv], sd = sdrate_[sv];
         const int tsv = a.type[sv];
-        const int dg = a.deg[sv];
-        uint32_t xs = s_lane ? xx_[(size_t)sv * S + lane] : 0u;       // lane s: count of species s
-        const double Dd = s_lane ? a.Ddiag[(size_t)sv * S + lane] : 0.0;
-        double rr = r_lane ? rrate_[(size_t)sv * R + lane] : 0.0;     // lane j: rate of reaction j
-        int nb_j = -1, nb_t = 1;
-        double nb_D = 0.0;
-        if (lane < W) {
-            const size_t e = (size_t)lane * n + sv;
-            nb_j = a.nidx[e];
-            nb_D = a.Dij[e];
-            if (T > 1) nb_t = a.ntype[e];
-        }
+        const int dg_l = a.deg[sv];
+        // lane s: count of species s; lane j: rate of reaction j; lanes past
+        // S / R / W read a valid element and drop it
+        const int ls = lane < S ? lane : S - 1, lr = lane < R ? lane : (R > 0 ? R - 1 : 0);
+        const int lw = lane < W ? lane : W - 1;
+        const uint32_t xs_raw = xx_[(size_t)sv * S + ls];
+        const double Dd_raw = a.Ddiag[(size_t)sv * S + ls];
+        const double rr_raw = R > 0 ? rrate_[(size_t)sv * R + lr] : 0.0;
+        const size_t e_l = (size_t)lw * n + sv;
+        const int nbj_raw = a.nidx[e_l];
+        const double nbD_raw = a.Dij[e_l];
+        const int nbt_raw = a.ntype[e_l];
+        uint32_t xs = s_lane ? xs_raw : 0u;
+        const double Dd = s_lane ? Dd_raw : 0.0;
+        double rr = r_lane ? rr_raw : 0.0;
+        const int nb_j = lane < W ? nbj_raw : -1;
+        const double nb_D = lane < W ? nbD_raw : 0.0;
+        const int nb_t = (lane < W && T > 1) ? nbt_raw : 1;
         double totrate = sr + sd;
         const double rand1 = rng.uniform(lane);
         if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
@@ -754,19 +839,27 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                 while (re < R && rv2 > cum) { re++; cum += re < R ? rl_d(rr, re) : 0.0; }
                 if (re >= R) { err = SSASDPD_ERR_PROPENSITY_OVERFLOW; break; }
             }
-            for (int k = L.jcN[re]; k < L.jcN[re + 1]; k++) {
-                const int sp = L.irN[k], d = L.prN[k];
-                if (lane == sp) { xs += (uint32_t)d; tot += (uint32_t)d; }
-                sd += rl_d(Dd, sp) * d;
+            {
+                // stoichiometry column re, ascending species (CSC order)
+                int d_l = 0;
+#pragma unroll
+                for (int r = 0; r < MR; r++)
+                    if (r == re) d_l = lm.dN[r];
+                uint64_t nz = __ballot(d_l != 0);
+                xs += (uint32_t)d_l;
+                tot += (uint32_t)d_l;
+                while (nz) {
+                    const int sp = (int)__builtin_ctzll(nz);
+                    nz &= nz - 1;
+                    sd += rl_d(Dd, sp) * __builtin_amdgcn_readlane(d_l, sp);
+                }
             }
-            double rd = 0.0;
-            for (int k = L.jcG[S + re]; k < L.jcG[S + re + 1]; k++) {
-                const int j = L.irG[k];
-                const double old = rl_d(rr, j);
-                const double p = propensity_l(L.rx[j], xs, vol, tsv);
-                if (lane == j) rr = p;
-                rd += p - old;
-            }
+            // dependency-graph column S+re: all propensities at once, the
+            // changes added in the column's order
+            const uint64_t dep = (uint32_t)__builtin_amdgcn_readlane((int)lm.dep, S + re);
+            const double p_new = propensity_par(lm, xs, vol, tsv);
+            const double rd = ordered_sum(0.0, p_new - rr, dep);
+            if ((dep >> lane) & 1ULL) rr = p_new;
             sr += rd;
             if (s_lane) xx_[(size_t)sv * S + lane] = xs;
             if (r_lane) rrate_[(size_t)sv * R + lane] = rr;
@@ -779,7 +872,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(a, qtime_, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(tsh, qtime_, L, sib, sv, ntv, lane);
         } else {
             // ---- diffusion (simulate_rdme.cpp:312-445)
             event = 1;
@@ -809,6 +902,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                     if (bad) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
                 }
             }
+            const int dg = __builtin_amdgcn_readfirstlane(dg_l);
             const double r2 = rng.uniform(lane);
             if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
             if (dg > WAVE) { err = SSASDPD_ERR_CAPACITY; break; }
@@ -880,7 +974,16 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (self) xd = xs;
             if (lane == spec) xd += 1u;
             if (self) xs = xd;
-            if (R > 0) {
+            if (R > 0 && !self) {
+                const uint64_t dep = (uint32_t)__builtin_amdgcn_readlane((int)lm.dep, spec);
+                const double p_s = propensity_par(lm, xs, vol, tsv);
+                const double p_d = propensity_par(lm, xd, vol, tds);
+                const double rd = ordered_sum(0.0, p_s - rr, dep);
+                const double rrdel = ordered_sum(0.0, p_d - rrd, dep);
+                if ((dep >> lane) & 1ULL) { rr = p_s; rrd = p_d; }
+                sr += rd;
+                srd += rrdel;
+            } else if (R > 0) {
                 double rd = 0.0, rrdel = 0.0;
                 for (int k = L.jcG[spec]; k < L.jcG[spec + 1]; k++) {
                     const int j = L.irG[k];
@@ -920,7 +1023,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(a, qtime_, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(tsh, qtime_, L, sib, sv, ntv, lane);
             const double dest_tot = srd + sdd;
             const double dold = self ? ntv : qtd;
             double ntd;
@@ -933,9 +1036,9 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             }
             if (sib.holds(dest)) {
                 sib.set(dest, ntd, lane);
-                tree_update_rescan<PL>(a, qtime_, L, sib, dest, ntd, lane);
+                tree_update_rescan<PL>(tsh, qtime_, L, sib, dest, ntd, lane);
             } else {
-                tree_update_point<PL>(a, qtime_, L, dest, ntd, lane);
+                tree_update_point<PL>(tsh, qtime_, L, dest, ntd, lane);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
