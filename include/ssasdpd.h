@@ -225,6 +225,35 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
  * computes no momentum there that any later step reads). */
 int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp, double *Frho);
 
+/* ---- Ensembles of different models ------------------------------------
+ * n_models static-domain models (uneven domain sizes, each its own
+ * chemistry), one trajectory each, with every event loop in ONE persistent
+ * launch that runs all nt steps (one workgroup per model, largest domains
+ * first).  The reference runs such a set as separate processes one after
+ * another (solver.py:547-605).  Model m's trajectory is bit-identical to
+ * ssasdpd_run of that model with rng {kind 0, seed seeds[m]}.  Models must
+ * share nt, dt and output steps.  Deterministic (PDE) species do not enter
+ * the event loop; an ensemble leaves them out and reports counts only. */
+typedef struct ssasdpd_ensemble ssasdpd_ensemble;
+int ssasdpd_ensemble_create(const ssasdpd_desc *const *descs, int32_t n_models, ssasdpd_ensemble **out);
+void ssasdpd_ensemble_destroy(ssasdpd_ensemble *m);
+/* Snapshot count K (the reference's output schedule). */
+int ssasdpd_ensemble_num_snapshots(ssasdpd_ensemble *m, int32_t *k);
+/* Event-tree width (voxels per lane under a level-1 node: 1, 2, 4, 8, 16 or
+ * 32) used by every model; results do not depend on it. */
+int ssasdpd_ensemble_set_tree_width(ssasdpd_ensemble *m, int32_t pl);
+/* Autotuner: times probe_steps steps of the ensemble (seeds as for run) at
+ * widths 1, 2, 4, 8 and keeps the fastest.  ms_per_width[4] (may be NULL)
+ * receives each probe's GPU time, -1 for widths not valid for every model. */
+int ssasdpd_ensemble_autotune(ssasdpd_ensemble *m, const uint64_t *seeds, int32_t probe_steps, int32_t *chosen,
+                              double *ms_per_width);
+/* Run all models: totals [n_models][K][S_max] species totals per snapshot
+ * (species past a model's S read 0), counts[m] (each may be NULL, the
+ * array itself may be NULL) [K][n_m][S_m]. */
+int ssasdpd_ensemble_run(ssasdpd_ensemble *m, const uint64_t *seeds, uint32_t *totals, uint32_t *const *counts);
+/* Events and times of the last run (ms_ssa: the persistent launch). */
+int ssasdpd_ensemble_stats(ssasdpd_ensemble *m, ssasdpd_stats *s);
+
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 
 /* GPU time per phase since the last reset, from HIP events on the engine's

@@ -131,11 +131,21 @@ def _declare(lib):
     lib.ssasdpd_mt19937_64_stream.argtypes = [C.c_uint64, C.c_uint64, _u64p, _dp]
     lib.ssasdpd_phase_times.argtypes = [P, _dp, _u64p, C.c_int32]
     lib.ssasdpd_eval_forces.argtypes = [P, C.c_int32, _dp, _dp, _dp]
+    lib.ssasdpd_ensemble_create.argtypes = [C.POINTER(C.POINTER(Desc)), C.c_int32, C.POINTER(P)]
+    lib.ssasdpd_ensemble_destroy.argtypes = [P]
+    lib.ssasdpd_ensemble_destroy.restype = None
+    lib.ssasdpd_ensemble_num_snapshots.argtypes = [P, _ip]
+    lib.ssasdpd_ensemble_set_tree_width.argtypes = [P, C.c_int32]
+    lib.ssasdpd_ensemble_autotune.argtypes = [P, _u64p, C.c_int32, _ip, _dp]
+    lib.ssasdpd_ensemble_run.argtypes = [P, _u64p, _up, C.POINTER(_up)]
+    lib.ssasdpd_ensemble_stats.argtypes = [P, C.POINTER(Stats)]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
                  "ssasdpd_run", "ssasdpd_run_batch", "ssasdpd_stats_get", "ssasdpd_get_state",
                  "ssasdpd_get_neighbors", "ssasdpd_fr_search", "ssasdpd_set_trace_capacity",
                  "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream", "ssasdpd_phase_times",
-                 "ssasdpd_eval_forces"]:
+                 "ssasdpd_eval_forces", "ssasdpd_ensemble_create", "ssasdpd_ensemble_num_snapshots",
+                 "ssasdpd_ensemble_set_tree_width", "ssasdpd_ensemble_autotune", "ssasdpd_ensemble_run",
+                 "ssasdpd_ensemble_stats"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

@@ -192,7 +192,7 @@ struct ssasdpd_engine {
                 if (ev_b[p][k]) (void)hipEventDestroy(ev_b[p][k]);
                 ev_a[p][k] = ev_b[p][k] = nullptr;
             }
-        if (st) (void)hipStreamDestroy(st);
+        if (st && !ext_stream) (void)hipStreamDestroy(st);
         st = nullptr;
     }
 
@@ -207,6 +207,51 @@ struct ssasdpd_engine {
         Dij = dalloc<double>(cap);
     }
 
+    // options set before setup (ensembles): a common event-tree width and a
+    // shared stream
+    int pl_override = 0;
+    hipStream_t ext_stream = nullptr;
+    int nnzN_ = 0, nnzG_ = 0;
+
+    static int auto_pl(int nv) {
+        int p = 1;
+        while (p < 32 && (nv + 64 * p - 1) / (64 * p) > 3072) p *= 2;
+        return p;
+    }
+    static bool pl_valid(int p, int nv) {
+        return (p == 1 || p == 2 || p == 4 || p == 8 || p == 16 || p == 32) && (nv + 64 * p - 1) / (64 * p) <= 3072;
+    }
+    // Level sizes/offsets for width p, tree storage and the event loop's LDS.
+    void layout_tree(int p) {
+        if (!pl_valid(p, n))
+            throw HipError("too many voxels for the event tree (max 6.2M)", hipErrorUnknown, __FILE__, __LINE__);
+        pl = p;
+        lev_n[0] = n;
+        lev_n[1] = (n + 64 * pl - 1) / (64 * pl);
+        lev_off[1] = 0;
+        nlev = 1;
+        tree_nodes = lev_n[1];
+        while (lev_n[nlev] > 1) {
+            lev_n[nlev + 1] = (lev_n[nlev] + TREE_FAN - 1) / TREE_FAN;
+            lev_off[nlev + 1] = tree_nodes;
+            tree_nodes += lev_n[nlev + 1];
+            nlev++;
+        }
+        dfree(tmin); dfree(targ);
+        tmin = dalloc<double>(tree_nodes);
+        targ = dalloc<int>(tree_nodes);
+        lds_bytes = ssa_lds_bytes(tree_nodes, S, R, T, nnzN_, nnzG_);
+        if (lds_bytes > 65536)
+            throw HipError("event-loop constants exceed 64 KiB of LDS", hipErrorUnknown, __FILE__, __LINE__);
+    }
+    // Change the tree width of a built queue: the levels are rebuilt from
+    // the firing times, so the event sequence is unchanged.
+    void set_pl(int p) {
+        if (p == pl) return;
+        layout_tree(p);
+        if (rdme_initialized) tree_rebuild(st, rdme_args(cur));
+    }
+
     void setup(const ssasdpd_desc *d) {
         n = d->n; dim = d->dimension; T = d->n_types; S = d->n_species; R = d->n_reactions;
         Sc = d->n_chem_species; Rc = d->n_chem_reactions; static_domain = d->static_domain;
@@ -215,7 +260,8 @@ struct ssasdpd_engine {
         for (int k = 0; k < 3; k++) g[k] = d->gravity[k];
         alpha = kernel_alpha(dim, h);
         out_steps.assign(d->output_steps, d->output_steps + d->n_output_steps);
-        HIP_OK(hipStreamCreate(&st));
+        if (ext_stream) st = ext_stream;
+        else HIP_OK(hipStreamCreate(&st));
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++) {
                 HIP_OK(hipEventCreate(&ev_a[p][k]));
@@ -286,35 +332,15 @@ struct ssasdpd_engine {
             qtime = dalloc<double>(n);
             // event tree: level 1 groups F0 = 64*pl voxels (pl chosen so the
             // levels above the voxels fit the event loop's LDS), 64 per node above
-            pl = 1;
-            while (pl < 32 && (n + 64 * pl - 1) / (64 * pl) > 3072) pl *= 2;
+            nnzN_ = R > 0 ? d->jcN[R] : 0;
+            nnzG_ = d->jcG[S + R];
+            int p0 = pl_override ? pl_override : auto_pl(n);
             if (const char *e = getenv("SSASDPD_EVENT_TREE_PL")) {
                 int want = atoi(e);
-                if (want == 1 || want == 2 || want == 4 || want == 8 || want == 16 || want == 32)
-                    if ((n + 64 * want - 1) / (64 * want) <= 3072) pl = want;
+                if (!pl_override && pl_valid(want, n)) p0 = want;
             }
-            if ((n + 64 * pl - 1) / (64 * pl) > 3072)
-                throw HipError("too many voxels for the event tree (max 6.2M)", hipErrorUnknown, __FILE__, __LINE__);
-            lev_n[0] = n;
-            lev_n[1] = (n + 64 * pl - 1) / (64 * pl);
-            lev_off[1] = 0;
-            nlev = 1;
-            tree_nodes = lev_n[1];
-            while (lev_n[nlev] > 1) {
-                lev_n[nlev + 1] = (lev_n[nlev] + TREE_FAN - 1) / TREE_FAN;
-                lev_off[nlev + 1] = tree_nodes;
-                tree_nodes += lev_n[nlev + 1];
-                nlev++;
-            }
-            tmin = dalloc<double>(tree_nodes);
-            targ = dalloc<int>(tree_nodes);
+            layout_tree(p0);
             vol = dalloc<double>(n);
-            {
-                int nnzN = R > 0 ? d->jcN[R] : 0, nnzG = d->jcG[S + R];
-                lds_bytes = ssa_lds_bytes(tree_nodes, S, R, T, nnzN, nnzG);
-                if (lds_bytes > 65536)
-                    throw HipError("event-loop constants exceed 64 KiB of LDS", hipErrorUnknown, __FILE__, __LINE__);
-            }
             qs = dalloc<QueueState>(1);
             qwork = (int *)dalloc<char>(sizeof(double) * (size_t)n + sizeof(int) * 4 * ((size_t)n + 1) + 64 +
                                         sizeof(double) * 512);
@@ -544,12 +570,32 @@ struct ssasdpd_engine {
         stats.max_degree = max_deg_seen;
     }
 
-    // Static domains without deterministic species: after step 0 the
-    // particle phase leaves the state unchanged (no motion, boundary
-    // conditions idempotent, rates and vol built once), so the event loops
-    // of all remaining steps run in one launch with in-kernel snapshots.
+    // Static domains: after step 0 the event loops of all remaining steps
+    // run in one launch with in-kernel snapshots (see particle_steps).
     bool fusable_static() const {
-        return static_domain && Sc == 0 && S > 0 && !(flags & SSASDPD_FLAG_NO_RDME) && cur > 0;
+        return static_domain && S > 0 && !(flags & SSASDPD_FLAG_NO_RDME) && cur > 0;
+    }
+
+    // Static domains after step 0: the particle phases of steps [s0, s1)
+    // alone.  The event loop neither reads nor writes anything they change
+    // (rates and vol are built once, rho does not evolve), so they run ahead
+    // of the fused event loop; without deterministic species they change
+    // nothing at all and are skipped.  snap(j) takes the field snapshot
+    // scheduled before step sst[j] (sst[j] == s1: after the last step).
+    template <class Snap>
+    int particle_steps(unsigned s0, unsigned s1, const std::vector<unsigned> &sst, Snap snap) {
+        size_t j = 0;
+        for (unsigned s = s0; s < s1; s++) {
+            while (j < sst.size() && sst[j] <= s) snap(j++);
+            if (Sc > 0) particle_phase(s);
+        }
+        while (j < sst.size()) snap(j++);
+        DevErr e = read_err();
+        if (e.code) {
+            stats.bad_particle = e.particle == INT32_MAX ? -1 : e.particle;
+            return set_error(e.code, "non-finite particle state in a static-domain step");
+        }
+        return 0;
     }
     bool fusable() const { return fusable_static() && rdme_initialized; }
 
@@ -582,6 +628,21 @@ struct ssasdpd_engine {
         cur = s_end;
         stats.steps = cur;
         stats.particle_steps = (uint64_t)cur * n;
+    }
+
+    // Step 0 of a static domain up to the event loop: neighbour lists, the
+    // step-0 particle phase, rates and the queue.  Returns whether the queue
+    // holds an event (the event loop then runs from step 0).
+    bool prepare_static() {
+        particle_phase(0);
+        RdmeArgs ra = rdme_args(0);
+        rdme_rates(st, ra);
+        rdme_queue_build(st, ra, scan_tmp, qwork);
+        QueueState q;
+        d2h(st, &q, qs, 1);
+        HIP_OK(hipStreamSynchronize(st));
+        rdme_initialized = q.initialized == 1;
+        return rdme_initialized;
     }
 
     int do_step() {
@@ -766,6 +827,117 @@ struct ssasdpd_engine {
 };
 
 // ---------------------------------------------------------------------------
+// Ensembles of different static models (uneven domains, own chemistry), one
+// trajectory each, all event loops in one persistent launch.
+struct ssasdpd_ensemble {
+    std::vector<ssasdpd_engine *> eng;
+    hipStream_t st = nullptr;
+    int M = 0, Smax = 0, max_sr = 0, pl = 1;
+    uint32_t nt = 0;
+    std::vector<unsigned> snap_steps;   // snapshot k is taken before step snap_steps[k]
+    std::vector<int> order;             // launch order: largest domain first
+    RdmeArgs *d_args = nullptr;
+    unsigned *d_snap = nullptr;
+    uint32_t *d_tot = nullptr;
+    ssasdpd_stats stats{};
+    double ms_launch = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    ~ssasdpd_ensemble() {
+        for (auto *e : eng) delete e;
+        dfree(d_args); dfree(d_snap); dfree(d_tot);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    size_t lds_max() const {
+        size_t m = 0;
+        for (auto *e : eng) m = std::max(m, e->lds_bytes);
+        return m;
+    }
+    bool pl_ok(int p) const {
+        for (auto *e : eng)
+            if (!ssasdpd_engine::pl_valid(p, e->n)) return false;
+        for (auto *e : eng)
+            if (ssa_lds_bytes_for(e, p) > 65536) return false;
+        return true;
+    }
+    static size_t ssa_lds_bytes_for(const ssasdpd_engine *e, int p) {
+        int n1 = (e->n + 64 * p - 1) / (64 * p), nodes = n1;
+        while (n1 > 1) { n1 = (n1 + TREE_FAN - 1) / TREE_FAN; nodes += n1; }
+        return ssa_lds_bytes(nodes, e->S, e->R, e->T, e->nnzN_, e->nnzG_);
+    }
+    void set_pl(int p) {
+        for (auto *e : eng) e->set_pl(p);
+        pl = p;
+    }
+    // Reset every model with its seed and build step 0 up to the event loop.
+    // init[m] = 0 for a model whose queue holds no event.
+    void prepare(const uint64_t *seeds, std::vector<char> &init) {
+        init.assign(M, 0);
+        for (int m = 0; m < M; m++) {
+            ssasdpd_rng r{};
+            r.kind = 0;
+            r.seed = seeds[m];
+            eng[m]->reset(&r);
+            init[m] = eng[m]->prepare_static() ? 1 : 0;
+        }
+    }
+    // One launch: steps [0, s_end) of every initialised model.
+    float launch(unsigned s_end, bool snapshots, std::vector<uint32_t *> *d_counts, const std::vector<char> &init) {
+        std::vector<RdmeArgs> args;
+        args.reserve(M);
+        const int K = (int)snap_steps.size();
+        for (int m : order) {
+            if (!init[m]) continue;
+            ssasdpd_engine *e = eng[m];
+            RdmeArgs a = e->rdme_args(0);
+            a.step = 0;
+            a.step_end = s_end;
+            a.t0 = 0.0;
+            a.trace = nullptr;
+            a.trace_cap = 0;
+            if (snapshots) {
+                a.snap_step = d_snap;
+                a.n_snap = K;
+                a.snap_tot = d_tot + (size_t)m * K * Smax;
+                a.snap_xx = d_counts ? (*d_counts)[m] : nullptr;
+            }
+            args.push_back(a);
+        }
+        if (args.empty()) return 0.f;
+        h2d(st, d_args, args.data(), args.size());
+        HIP_OK(hipEventRecord(e0, st));
+        rdme_take_step_multi(st, d_args, (int)args.size(), pl, max_sr, lds_max());
+        HIP_OK(hipEventRecord(e1, st));
+        HIP_OK(hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    }
+    int collect(const std::vector<char> &init, unsigned s_end) {
+        stats.reactions = stats.diffusions = stats.rng_words_used = 0;
+        int rc = 0;
+        for (int m = 0; m < M; m++) {
+            if (!init[m]) continue;
+            QueueState q;
+            d2h(st, &q, eng[m]->qs, 1);
+            HIP_OK(hipStreamSynchronize(st));
+            stats.reactions += q.reactions;
+            stats.diffusions += q.diffusions;
+            stats.rng_words_used += q.rng_pos;
+            if (q.err && !rc) {
+                char buf[128];
+                snprintf(buf, sizeof buf, "SSA failed at step %u in model %d", q.err_step, m);
+                rc = set_error(q.err, buf);
+            }
+            eng[m]->cur = s_end;
+        }
+        return rc;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // C ABI
 #define GUARD_BEGIN try {
 #define GUARD_END                                                            \
@@ -878,8 +1050,10 @@ int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps) {
     const uint32_t target = (uint32_t)std::min<uint64_t>((uint64_t)e->cur + n_steps, e->nt);
     while (e->cur < target) {
         if (e->fusable()) {
+            int rc = e->particle_steps(e->cur, target, {}, [](size_t) {});
+            if (rc) return rc;
             e->ssa_fused(e->rdme_args(e->cur), 1, target, {}, nullptr, 0, nullptr);
-            int rc = e->check(target - 1);
+            rc = e->check(target - 1);
             if (rc) return rc;
             break;
         }
@@ -926,17 +1100,20 @@ int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
             const int K = o ? (int)std::count_if(ks.begin(), ks.end(), [&](int q) { return q < o->n_snapshots; }) : 0;
             sst.resize(K);
             const size_t per = (size_t)e->n * e->S;
-            uint32_t *d_xx = (K > 0 && o->xx) ? dalloc<uint32_t>(per * K) : nullptr;
-            e->ssa_fused(e->rdme_args(step), 1, e->nt, sst, d_xx, 0, nullptr);
-            int rc = e->check(e->nt - 1);
-            for (int j = 0; j < K && !rc; j++) {
-                if (d_xx) d2h(e->st, o->xx + (size_t)ks[j] * per, d_xx + (size_t)j * per, per);
+            // particle fields first (independent of the event loop here)
+            int rc = e->particle_steps(step, e->nt, sst, [&](size_t j) {
                 uint32_t *keep = o->xx;
                 o->xx = nullptr;
-                e->snapshot(o, ks[j]);        // particle fields: constant after step 0
+                e->snapshot(o, ks[j]);
                 o->xx = keep;
                 if (o->steps) o->steps[ks[j]] = sst[j];
-            }
+            });
+            if (rc) return rc;
+            uint32_t *d_xx = (K > 0 && o->xx) ? dalloc<uint32_t>(per * K) : nullptr;
+            e->ssa_fused(e->rdme_args(step), 1, e->nt, sst, d_xx, 0, nullptr);
+            rc = e->check(e->nt - 1);
+            for (int j = 0; j < K && !rc; j++)
+                if (d_xx) d2h(e->st, o->xx + (size_t)ks[j] * per, d_xx + (size_t)j * per, per);
             HIP_OK(hipStreamSynchronize(e->st));
             dfree(d_xx);
             if (rc) return rc;
@@ -1002,11 +1179,12 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
             std::vector<int> ks;
             std::vector<unsigned> sst = plan_snapshots(e, step, next, idx, k, ks);
             const int K = (int)sst.size();
+            int rc = e->particle_steps(step, e->nt, sst, [&](size_t j) { snap_shared(ks[j], sst[j]); });
+            if (rc) return rc;
             uint32_t *d_xx = (counts && e->S > 0) ? dalloc<uint32_t>(per * K * B) : nullptr;
             e->ssa_fused(e->batch_args(step, -1), B, e->nt, sst, d_xx, per * K, nullptr);
-            int rc = e->batch_check(B, e->nt - 1);
+            rc = e->batch_check(B, e->nt - 1);
             for (int j = 0; j < K && !rc; j++) {
-                snap_shared(ks[j], sst[j]);
                 if (d_xx)
                     for (int b = 0; b < B; b++)
                         d2h(e->st, counts + ((size_t)b * n_snap + ks[j]) * per, d_xx + ((size_t)b * K + j) * per, per);
@@ -1058,6 +1236,166 @@ int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp,
     dfree(dF); dfree(dB); dfree(dR); dfree(dQ);
     return 0;
     GUARD_END
+}
+
+int ssasdpd_ensemble_create(const ssasdpd_desc *const *descs, int32_t n_models, ssasdpd_ensemble **out) {
+    GUARD_BEGIN
+    if (!descs || n_models <= 0 || !out) return set_error(SSASDPD_ERR_BAD_DESC, "ensemble: no models");
+    for (int m = 0; m < n_models; m++) {
+        int rc = validate(descs[m]);
+        if (rc) return rc;
+        const ssasdpd_desc *d = descs[m];
+        if (!d->static_domain) return set_error(SSASDPD_ERR_UNSUPPORTED, "ensemble: static domains only");
+        if (d->n_species <= 0) return set_error(SSASDPD_ERR_BAD_DESC, "ensemble: every model needs species");
+        const ssasdpd_desc *d0 = descs[0];
+        bool same = d->nt == d0->nt && d->dt == d0->dt && d->n_output_steps == d0->n_output_steps;
+        for (int k = 0; same && k < d->n_output_steps; k++) same = d->output_steps[k] == d0->output_steps[k];
+        if (!same) return set_error(SSASDPD_ERR_BAD_DESC, "ensemble: models must share nt, dt and output steps");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(SSASDPD_ERR_HIP, "no HIP device");
+    auto *E = new ssasdpd_ensemble();
+    try {
+        E->M = n_models;
+        HIP_OK(hipStreamCreate(&E->st));
+        HIP_OK(hipEventCreate(&E->e0));
+        HIP_OK(hipEventCreate(&E->e1));
+        int p = 1;
+        for (int m = 0; m < n_models; m++) p = std::max(p, ssasdpd_engine::auto_pl(descs[m]->n));
+        for (int m = 0; m < n_models; m++) {
+            // the deterministic species do not enter the event loop, and an
+            // ensemble reports counts only, so they are left out
+            ssasdpd_desc dm = *descs[m];
+            dm.n_chem_species = 0;
+            dm.n_chem_reactions = 0;
+            auto *e = new ssasdpd_engine();
+            e->pl_override = p;
+            e->ext_stream = E->st;
+            E->eng.push_back(e);
+            e->setup(&dm);
+            E->Smax = std::max(E->Smax, e->S);
+            E->max_sr = std::max(E->max_sr, std::max(e->S, e->R));
+        }
+        E->pl = p;
+        E->nt = descs[0]->nt;
+        uint32_t next = 0;
+        size_t idx = 0;
+        int k = 0;
+        std::vector<int> ks;
+        E->snap_steps = plan_snapshots(E->eng[0], 0, next, idx, k, ks);
+        E->order.resize(n_models);
+        for (int m = 0; m < n_models; m++) E->order[m] = m;
+        std::stable_sort(E->order.begin(), E->order.end(),
+                         [&](int a, int b) { return E->eng[a]->n > E->eng[b]->n; });
+        E->d_args = dalloc<RdmeArgs>(n_models);
+        E->d_snap = dalloc<unsigned>(E->snap_steps.size());
+        h2d(E->st, E->d_snap, E->snap_steps.data(), E->snap_steps.size());
+        E->d_tot = dalloc<uint32_t>((size_t)n_models * E->snap_steps.size() * E->Smax);
+        HIP_OK(hipStreamSynchronize(E->st));
+    } catch (...) {
+        delete E;
+        throw;
+    }
+    *out = E;
+    return 0;
+    GUARD_END
+}
+
+void ssasdpd_ensemble_destroy(ssasdpd_ensemble *m) { delete m; }
+
+int ssasdpd_ensemble_num_snapshots(ssasdpd_ensemble *m, int32_t *k) {
+    *k = (int32_t)m->snap_steps.size();
+    return 0;
+}
+
+int ssasdpd_ensemble_set_tree_width(ssasdpd_ensemble *m, int32_t pl) {
+    GUARD_BEGIN
+    if (!m->pl_ok(pl)) return set_error(SSASDPD_ERR_BAD_DESC, "ensemble: tree width not valid for every model");
+    m->set_pl(pl);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_ensemble_autotune(ssasdpd_ensemble *m, const uint64_t *seeds, int32_t probe_steps, int32_t *chosen,
+                              double *ms_per_width) {
+    GUARD_BEGIN
+    const int cand[4] = {1, 2, 4, 8};
+    const unsigned s_end = (unsigned)std::max(1, std::min<int>(probe_steps, (int)m->nt));
+    float best = 3.4e38f;
+    int best_pl = m->pl;
+    for (int c = 0; c < 4; c++) {
+        if (ms_per_width) ms_per_width[c] = -1.0;
+        if (!m->pl_ok(cand[c])) continue;
+        m->set_pl(cand[c]);
+        std::vector<char> init;
+        m->prepare(seeds, init);
+        const float ms = m->launch(s_end, false, nullptr, init);
+        int rc = m->collect(init, s_end);
+        if (rc) return rc;
+        if (ms_per_width) ms_per_width[c] = ms;
+        if (ms < best) { best = ms; best_pl = cand[c]; }
+    }
+    m->set_pl(best_pl);
+    if (chosen) *chosen = best_pl;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_ensemble_run(ssasdpd_ensemble *m, const uint64_t *seeds, uint32_t *totals, uint32_t *const *counts) {
+    GUARD_BEGIN
+    if (!seeds) return set_error(SSASDPD_ERR_BAD_DESC, "ensemble_run: seeds required");
+    auto t0 = std::chrono::steady_clock::now();
+    const int K = (int)m->snap_steps.size();
+    std::vector<char> init;
+    m->prepare(seeds, init);
+    std::vector<uint32_t *> d_counts;
+    if (counts) {
+        d_counts.assign(m->M, nullptr);
+        for (int q = 0; q < m->M; q++)
+            if (counts[q]) d_counts[q] = dalloc<uint32_t>((size_t)K * m->eng[q]->n * m->eng[q]->S);
+    }
+    m->ms_launch = m->launch(m->nt, true, counts ? &d_counts : nullptr, init);
+    int rc = m->collect(init, m->nt);
+    if (!rc) {
+        std::vector<uint32_t> tot((size_t)m->M * K * m->Smax);
+        d2h(m->st, tot.data(), m->d_tot, tot.size());
+        HIP_OK(hipStreamSynchronize(m->st));
+        for (int q = 0; q < m->M; q++) {
+            ssasdpd_engine *e = m->eng[q];
+            const size_t per = (size_t)e->n * e->S;
+            std::vector<uint32_t> u0sum(e->S, 0);
+            if (!init[q])
+                for (size_t i = 0; i < per; i++) u0sum[i % e->S] += e->u0[i];
+            for (int k = 0; k < K; k++)
+                for (int s = 0; s < m->Smax; s++) {
+                    uint32_t v = 0;
+                    if (s < e->S) v = init[q] ? tot[((size_t)q * K * m->Smax) + (size_t)k * e->S + s] : u0sum[s];
+                    if (totals) totals[((size_t)q * K + k) * m->Smax + s] = v;
+                }
+            if (counts && counts[q]) {
+                if (init[q]) {
+                    d2h(m->st, counts[q], d_counts[q], per * K);
+                } else {
+                    for (int k = 0; k < K; k++) std::copy(e->u0.begin(), e->u0.end(), counts[q] + (size_t)k * per);
+                }
+            }
+        }
+        HIP_OK(hipStreamSynchronize(m->st));
+    }
+    for (auto *p : d_counts) dfree(p);
+    m->stats.steps = m->nt;
+    uint64_t ps = 0;
+    for (auto *e : m->eng) ps += (uint64_t)e->n * m->nt;
+    m->stats.particle_steps = ps;
+    m->stats.ms_ssa = m->ms_launch;
+    m->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+    GUARD_END
+}
+
+int ssasdpd_ensemble_stats(ssasdpd_ensemble *m, ssasdpd_stats *s) {
+    *s = m->stats;
+    return 0;
 }
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s) {

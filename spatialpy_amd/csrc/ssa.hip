@@ -905,50 +905,76 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             const int dg = __builtin_amdgcn_readfirstlane(dg_l);
             const double r2 = rng.uniform(lane);
             if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
-            if (dg > WAVE) { err = SSASDPD_ERR_CAPACITY; break; }
-            // ---- round trip 2: every neighbour's state (lane k fetches
-            // neighbour k into LDS) issued before the walk picks one
-            const bool have = lane < dg && nb_j >= 0;
+            double rand2 = r2 * rl_d(Dd, spec);
+            int kk = -1;             // chosen entry of the neighbour row
+            bool pref = false;       // its state is in the prefetch buffers
             double n_sr = 0.0, n_sd = 0.0, n_Dd = 0.0, n_qt = inf;
             int n_ty = 1;
-            if (have) {
-                const size_t j = (size_t)nb_j;
+            if (dg <= WAVE) {
+                // ---- round trip 2: every neighbour's state (lane k fetches
+                // neighbour k into LDS) issued before the walk picks one
+                const bool have = lane < dg && nb_j >= 0;
+                if (have) {
+                    const size_t j = (size_t)nb_j;
 #pragma unroll
-                for (int s2 = 0; s2 < MS; s2++)
-                    if (s2 < S) nbx[lane * MS + s2] = xx_[j * S + s2];
+                    for (int s2 = 0; s2 < MS; s2++)
+                        if (s2 < S) nbx[lane * MS + s2] = xx_[j * S + s2];
 #pragma unroll
-                for (int r = 0; r < MR; r++)
-                    if (r < R) nbr[lane * MR + r] = rrate_[j * R + r];
-                n_sr = srrate_[j];
-                n_sd = sdrate_[j];
-                n_Dd = a.Ddiag[j * S + spec];
-                n_qt = qtime_[j];
-                n_ty = a.type[j];
-            }
-            double rand2 = r2 * rl_d(Dd, spec);
-            // destination: walk the neighbour list in order
-            const double c = have ? nb_D * L.Dmat[spec * T + (nb_t - 1)] : 0.0;
-            int kk = -1;
-            const int m = dg < WAVE ? dg : WAVE;
-            for (int pass = 0; pass < 2 && kk < 0; pass++) {
-                double cum2 = 0.0;
-                for (int k = 0; k < m; k++) {
-                    cum2 += rl_d(c, k);
-                    if (cum2 > rand2) { kk = k; break; }
+                    for (int r = 0; r < MR; r++)
+                        if (r < R) nbr[lane * MR + r] = rrate_[j * R + r];
+                    n_sr = srrate_[j];
+                    n_sd = sdrate_[j];
+                    n_Dd = a.Ddiag[j * S + spec];
+                    n_qt = qtime_[j];
+                    n_ty = a.type[j];
                 }
-                if (kk < 0) {
-                    if (dest >= 0) break;   // stale destination of an earlier event (simulate_rdme.cpp:368)
-                    rand2 = r2 * cum2;
+                // destination: walk the neighbour list in order
+                const double c = have ? nb_D * L.Dmat[spec * T + (nb_t - 1)] : 0.0;
+                for (int pass = 0; pass < 2 && kk < 0; pass++) {
+                    double cum2 = 0.0;
+                    for (int k = 0; k < dg; k++) {
+                        cum2 += rl_d(c, k);
+                        if (cum2 > rand2) { kk = k; break; }
+                    }
+                    if (kk < 0) {
+                        if (dest >= 0) break;   // stale destination of an earlier event (simulate_rdme.cpp:368)
+                        rand2 = r2 * cum2;
+                    }
+                }
+                pref = kk >= 0;
+            } else {
+                // rows longer than a wavefront: the same in-order walk, 64
+                // entries at a time; the destination's state is loaded after
+                for (int pass = 0; pass < 2 && kk < 0; pass++) {
+                    double cum2 = 0.0;
+                    for (int c0 = 0; c0 < dg && kk < 0; c0 += WAVE) {
+                        const int k = c0 + lane;
+                        double cl = 0.0;
+                        if (k < dg) {
+                            const size_t e = (size_t)k * n + sv;
+                            const int tk = T > 1 ? a.ntype[e] : 1;
+                            cl = a.Dij[e] * L.Dmat[spec * T + (tk - 1)];
+                        }
+                        const int mm = dg - c0 < WAVE ? dg - c0 : WAVE;
+                        for (int q2 = 0; q2 < mm; q2++) {
+                            cum2 += rl_d(cl, q2);
+                            if (cum2 > rand2) { kk = c0 + q2; break; }
+                        }
+                    }
+                    if (kk < 0) {
+                        if (dest >= 0) break;
+                        rand2 = r2 * cum2;
+                    }
                 }
             }
             if (kk < 0 && dest < 0) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             // destination state (lane s / lane j layout): from the prefetch,
-            // or loaded when a stale destination is not in this row
+            // or loaded (long rows, or a stale destination not in this row)
             uint32_t xd;
             double rrd, srd, sdd, Ddd, qtd;
             int tds;
-            if (kk >= 0) {
+            if (pref) {
                 dest = __builtin_amdgcn_readlane(nb_j, kk);
                 xd = s_lane ? nbx[kk * MS + lane] : 0u;
                 rrd = r_lane ? nbr[kk * MR + lane] : 0.0;
@@ -958,6 +984,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                 qtd = rl_d(n_qt, kk);
                 tds = __builtin_amdgcn_readlane(n_ty, kk);
             } else {
+                if (kk >= 0) dest = __builtin_amdgcn_readfirstlane(a.nidx[(size_t)kk * n + sv]);
                 const size_t j = (size_t)dest;
                 xd = s_lane ? xx_[j * S + lane] : 0u;
                 rrd = r_lane ? rrate_[j * R + lane] : 0.0;
@@ -1075,7 +1102,7 @@ __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
 // Ensembles of different models: workgroup g runs the trajectory described
 // by args[g] (its own domain, chemistry, queue and stream).
 template <int PL, int MS, int MR>
-__global__ void __launch_bounds__(WAVE) k_ssa_multi(const RdmeArgs *args) {
+__global__ void __launch_bounds__(WAVE) k_ssa_multi(const RdmeArgs *__restrict__ args) {
     ssa_run<PL, MS, MR>(args[blockIdx.x], 0);
 }
 

@@ -72,6 +72,27 @@ def test_c1_replay_matches_oracle_event_by_event(oracle):
     assert st.rng_words_used == ost.rng_words_used
 
 
+@pytest.mark.parametrize("hfac", [4.7, 6.5])
+def test_long_neighbour_rows_replay_matches_oracle(hfac, oracle):
+    """Rows longer than one wavefront (about 65 and 130 neighbours): the
+    diffusion walk runs 64 entries at a time; event-by-event parity."""
+    sa = workloads.birth_death_2d(seed=3, t_end=2, num_points=3)
+    sa.h = hfac * (sa.x[1, 1] - sa.x[0, 1] if sa.x[1, 1] != sa.x[0, 1] else 0.1)
+    words, expo = mt19937_64_stream(11, 400_000)
+    eng = Engine(sa, trace_capacity=100_000)
+    res = eng.run(words=words, expo=expo)
+    rp = eng.neighbors()[0]
+    assert np.diff(rp).max() > 64
+    ev, nev = eng.trace(100_000)
+    o = oracle.Oracle(sa)
+    o.set_trace(100_000)
+    ref = o.run(seed=11)
+    oev, onev = o.trace(100_000)
+    assert nev == onev and nev > 500
+    assert np.array_equal(ev, oev)
+    assert np.array_equal(res["xx"], ref["xx"])
+
+
 def test_c2_small_replay_matches_oracle(oracle):
     sa = workloads.static_rd_3d(n_side=12, nA=3000, nB=3000, t_end=3, num_points=4, seed=2)
     words, expo = mt19937_64_stream(11, 200_000)
