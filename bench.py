@@ -75,6 +75,32 @@ def cpu_baseline(sa, steps, seconds_budget=30.0):
             "sample": f"FP64 oracle restatement, 1 thread, {sa.n} particles, {steps} steps: {sec:.2f} s"}
 
 
+# matrix-core flops of one surrogate forward pass per node: encoder 32->64,
+# per layer A|B 64->128, W2 64->64, Wn1 128->64, Wn2 64->64 (x 3 layers)
+SURROGATE_FLOPS_PER_NODE = 2 * (32 * 64) + 3 * 2 * (64 * 128 + 64 * 64 + 128 * 64 + 64 * 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_timing(eng, sa, plain_pair_ms):
+    """GPU time of one forward pass of the message-passing surrogate on the
+    same 1M-particle state (weights untrained: the timing does not depend on
+    them), next to the plain pair-force kernel it would replace."""
+    from spatialpy_amd.surrogate import Surrogate, fit_scales
+    s = Surrogate(seed=0)
+    s.set_scales(*fit_scales(eng, sa))
+    idx = s.add_engine_sample(eng)
+    ms = s.time_forward(idx, reps=5)
+    tflops = SURROGATE_FLOPS_PER_NODE * sa.n / (ms * 1e-3) / 1e12
+    s.close()
+    return {"forward_ms": round(ms, 4), "plain_pair_kernel_ms": round(plain_pair_ms, 4),
+            "speed_vs_plain_pair_kernel": round(plain_pair_ms / ms, 4),
+            "mfma": {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5),
+                     "flops_per_node": SURROGATE_FLOPS_PER_NODE},
+            "note": "bf16 MFMA GEMMs + FP32 edge sums; accuracy is measured by the GPU tests "
+                    "(tests/test_gpu_mpn.py), not here"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +113,22 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate forward-pass timing")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with torch (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nb, "dispatches": 3}))
+        return
     if args.warmup + args.steps > 20:
         print("warning: this workload turns non-finite at step 21 (also in the reference engine); "
               "keep warmup + steps <= 20", file=sys.stderr)
@@ -164,6 +205,8 @@ def main():
         "roofline_particle_path": particle_roof,
         "roofline_ssa": ssa_roof,
     }
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_timing(eng, sa, dph["forces"][0] / args.steps)
     if not args.no_cpu_baseline:
         from spatialpy_amd import workloads
         csa = build_workload(args.n, args.cpu_steps + 1)

@@ -254,6 +254,56 @@ int ssasdpd_ensemble_run(ssasdpd_ensemble *m, const uint64_t *seeds, uint32_t *t
 /* Events and times of the last run (ms_ssa: the persistent launch). */
 int ssasdpd_ensemble_stats(ssasdpd_ensemble *m, ssasdpd_stats *s);
 
+/* ---- Learned surrogate for the SDPD accelerations ------------------------
+ * No counterpart in the reference.  A message-passing network over the
+ * neighbour graph (3 layers; edge MLP 132->64->64 on [h_i, h_j, dx/h, r/h]
+ * summed over neighbours, node MLP 128->64->64, hidden width 64, SiLU, bf16
+ * weights, FP32 accumulation on the matrix cores) predicts each moving
+ * particle's pair acceleration F - g.  Node features: v / v_scale,
+ * (rho/rho0 - 1) / rho_scale, solid flag, nu / nu_scale, 1, (vt - v) / v_scale.  Training runs
+ * on the GPU (forward, backward, Adam); the engine uses the network between
+ * periodic checks against the plain SDPD and falls back to it for the rest
+ * of the run when the relative error exceeds the budget.  Reactions and
+ * diffusion stay exact SSA. */
+typedef struct ssasdpd_mpn ssasdpd_mpn;
+int32_t ssasdpd_mpn_num_params(void);
+int ssasdpd_mpn_create(uint64_t seed, ssasdpd_mpn **out);
+void ssasdpd_mpn_destroy(ssasdpd_mpn *m);
+int ssasdpd_mpn_get_params(ssasdpd_mpn *m, float *params);
+int ssasdpd_mpn_set_params(ssasdpd_mpn *m, const float *params);
+/* scales[5] = {rho0, v_scale, rho_scale, nu_scale, acc_scale} */
+int ssasdpd_mpn_set_scales(ssasdpd_mpn *m, const float *scales);
+/* One training graph: x [n][3], node features X0 [n][32], ELL lists deg [n]
+ * and nidx [W][n], targets y [n][3] (normalised F - g). */
+int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, int32_t n, int32_t W, float inv_h, const float *x, const float *X0,
+                           const int32_t *deg, const int32_t *nidx, const float *y, int32_t *index);
+/* A training graph from an engine's current state, with the plain pair
+ * kernel's accelerations on that state as the target. */
+int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t *index);
+int ssasdpd_mpn_clear_samples(ssasdpd_mpn *m);
+/* Record a training graph every `every` steps of the engine's plain SDPD
+ * steps (m NULL: stop): the state the pair kernel sees inside the step
+ * (after the predictor) and its accelerations.  Scales must be set first. */
+int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every);
+int32_t ssasdpd_mpn_num_samples(ssasdpd_mpn *m);
+/* Network output [n][3] (normalised) for a stored graph. */
+int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t sample, float *out);
+/* GPU time of one forward pass over a stored graph (HIP events, reps runs). */
+int ssasdpd_mpn_time_forward(ssasdpd_mpn *m, int32_t sample, int32_t reps, double *ms_per_forward);
+/* Mean squared error of a stored graph and its gradient [num_params]. */
+int ssasdpd_mpn_grad(ssasdpd_mpn *m, int32_t sample, float *grad, double *loss);
+/* Adam, one step per stored graph per epoch (shuffled); mean loss of the
+ * last epoch. */
+int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle_seed, double *mean_loss);
+/* Attach (m != NULL) or detach the surrogate: on moving domains every step
+ * whose index is not a multiple of check_every takes F from the network;
+ * the others run the plain kernel and compare (relative L2 error of F - g
+ * over moving particles), and an error above max_rel_err switches the
+ * network off until the next reset or attach.  Counters restart at 0. */
+int ssasdpd_set_surrogate(ssasdpd_engine *e, ssasdpd_mpn *m, int32_t check_every, double max_rel_err);
+int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *checks, double *last_err,
+                            int32_t *active);
+
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 
 /* GPU time per phase since the last reset, from HIP events on the engine's

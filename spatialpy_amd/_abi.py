@@ -139,13 +139,37 @@ def _declare(lib):
     lib.ssasdpd_ensemble_autotune.argtypes = [P, _u64p, C.c_int32, _ip, _dp]
     lib.ssasdpd_ensemble_run.argtypes = [P, _u64p, _up, C.POINTER(_up)]
     lib.ssasdpd_ensemble_stats.argtypes = [P, C.POINTER(Stats)]
+    _fp = C.POINTER(C.c_float)
+    lib.ssasdpd_mpn_num_params.restype = C.c_int32
+    lib.ssasdpd_mpn_create.argtypes = [C.c_uint64, C.POINTER(P)]
+    lib.ssasdpd_mpn_destroy.argtypes = [P]
+    lib.ssasdpd_mpn_destroy.restype = None
+    lib.ssasdpd_mpn_get_params.argtypes = [P, _fp]
+    lib.ssasdpd_mpn_set_params.argtypes = [P, _fp]
+    lib.ssasdpd_mpn_set_scales.argtypes = [P, _fp]
+    lib.ssasdpd_mpn_add_sample.argtypes = [P, C.c_int32, C.c_int32, C.c_float, _fp, _fp, _ip, _ip, _fp, _ip]
+    lib.ssasdpd_mpn_add_engine_sample.argtypes = [P, P, _ip]
+    lib.ssasdpd_mpn_clear_samples.argtypes = [P]
+    lib.ssasdpd_mpn_forward.argtypes = [P, C.c_int32, _fp]
+    lib.ssasdpd_mpn_grad.argtypes = [P, C.c_int32, _fp, _dp]
+    lib.ssasdpd_mpn_train.argtypes = [P, C.c_int32, C.c_float, C.c_uint64, _dp]
+    lib.ssasdpd_set_surrogate.argtypes = [P, P, C.c_int32, C.c_double]
+    lib.ssasdpd_mpn_record.argtypes = [P, P, C.c_int32]
+    lib.ssasdpd_mpn_time_forward.argtypes = [P, C.c_int32, C.c_int32, _dp]
+    lib.ssasdpd_mpn_num_samples.argtypes = [P]
+    lib.ssasdpd_mpn_num_samples.restype = C.c_int32
+    lib.ssasdpd_surrogate_stats.argtypes = [P, _u64p, _u64p, _dp, _ip]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
                  "ssasdpd_run", "ssasdpd_run_batch", "ssasdpd_stats_get", "ssasdpd_get_state",
                  "ssasdpd_get_neighbors", "ssasdpd_fr_search", "ssasdpd_set_trace_capacity",
                  "ssasdpd_get_trace", "ssasdpd_mt19937_64_stream", "ssasdpd_phase_times",
                  "ssasdpd_eval_forces", "ssasdpd_ensemble_create", "ssasdpd_ensemble_num_snapshots",
                  "ssasdpd_ensemble_set_tree_width", "ssasdpd_ensemble_autotune", "ssasdpd_ensemble_run",
-                 "ssasdpd_ensemble_stats"]:
+                 "ssasdpd_ensemble_stats", "ssasdpd_mpn_create", "ssasdpd_mpn_get_params",
+                 "ssasdpd_mpn_set_params", "ssasdpd_mpn_set_scales", "ssasdpd_mpn_add_sample",
+                 "ssasdpd_mpn_add_engine_sample", "ssasdpd_mpn_clear_samples", "ssasdpd_mpn_forward",
+                 "ssasdpd_mpn_grad", "ssasdpd_mpn_train", "ssasdpd_set_surrogate", "ssasdpd_surrogate_stats",
+                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

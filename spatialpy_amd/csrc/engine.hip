@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <random>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "mpn.hpp"
 
 using namespace ssa;
 
@@ -170,6 +172,7 @@ struct ssasdpd_engine {
     ~ssasdpd_engine() { release(); }
 
     void release() {
+        dfree(nn_X0); dfree(nn_xf); dfree(nn_F); dfree(nn_acc);
         dfree(b_xx); dfree(b_rr); dfree(b_sr); dfree(b_sd); dfree(b_qt); dfree(b_tmin); dfree(b_targ);
         dfree(b_qs); dfree(b_seeds);
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
@@ -212,6 +215,61 @@ struct ssasdpd_engine {
     int pl_override = 0;
     hipStream_t ext_stream = nullptr;
     int nnzN_ = 0, nnzG_ = 0;
+    // surrogate for the pair accelerations (null: plain SDPD)
+    Mpn *nn = nullptr;
+    int nn_check_every = 50;
+    double nn_tol = 0.02, nn_last_err = -1.0;
+    bool nn_active = false;
+    uint64_t nn_steps = 0, nn_checks = 0;
+    float *nn_X0 = nullptr, *nn_xf = nullptr;
+    Mpn *nn_rec = nullptr;         // record training graphs during plain steps
+    int nn_rec_every = 0;
+    double *nn_F = nullptr, *nn_acc = nullptr;
+    int nn_cap = 0;
+
+    MpnGraph nn_graph() {
+        MpnGraph gr;
+        gr.n = n; gr.W = W; gr.inv_h = (float)(1.0 / h); gr.x = nn_xf; gr.deg = deg; gr.nidx = nidx;
+        return gr;
+    }
+    void nn_ensure() {
+        if (nn_cap >= n) return;
+        dfree(nn_X0); dfree(nn_xf); dfree(nn_F); dfree(nn_acc);
+        nn_X0 = dalloc<float>((size_t)n * 32);
+        nn_xf = dalloc<float>((size_t)n * 3);
+        nn_F = dalloc<double>((size_t)n * 3);
+        nn_acc = dalloc<double>(2);
+        nn_cap = n;
+    }
+    // A training graph from the state the force kernel just saw (after the
+    // predictor, lists of this step) with its accelerations as the target.
+    void record_sample() {
+        Mpn *M = nn_rec;
+        MpnSample smp;
+        smp.n = n; smp.W = W; smp.inv_h = (float)(1.0 / h);
+        smp.x = dalloc<float>((size_t)n * 3); smp.X0 = dalloc<float>((size_t)n * 32);
+        smp.deg = dalloc<int>(n); smp.nidx = dalloc<int>((size_t)n * W); smp.y = dalloc<float>((size_t)n * 3);
+        mpn_features(st, n, x, v, vt, rho, nu, solid, M->rho0, M->v_scale, M->rho_scale, M->nu_scale, smp.X0, smp.x);
+        mpn_target(st, n, F, solid, g, M->acc_scale, smp.y);
+        HIP_OK(hipMemcpyAsync(smp.deg, deg, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(smp.nidx, nidx, sizeof(int) * (size_t)n * W, hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));
+        M->samples.push_back(smp);
+    }
+
+    // the network's accelerations for the current state into Fout (SoA
+    // [3][n]); the caller has run nn_ensure(), so nn_F is allocated before it
+    // is passed here
+    void nn_predict(double *Fout) {
+        if (!Fout || nn_cap < n)
+            throw HipError("surrogate buffers not allocated", hipErrorInvalidValue, __FILE__, __LINE__);
+        mpn_features(st, n, x, v, vt, rho, nu, solid, nn->rho0, nn->v_scale, nn->rho_scale, nn->nu_scale, nn_X0,
+                     nn_xf);
+        HIP_OK(hipStreamSynchronize(st));
+        nn->forward(nn_graph(), nn_X0, nullptr);
+        HIP_OK(hipStreamSynchronize(nn->st));
+        mpn_apply(st, n, dim, nn->out_buf(), solid, g, nn->acc_scale, Fout);
+    }
 
     static int auto_pl(int nv) {
         int p = 1;
@@ -412,6 +470,9 @@ struct ssasdpd_engine {
         max_deg_seen = 0;
         memset(&stats, 0, sizeof(stats));
         stats.bad_particle = -1;
+        nn_active = nn != nullptr;
+        nn_steps = nn_checks = 0;
+        nn_last_err = -1.0;
         for (int p = 0; p < NPH; p++) { ph_ms[p] = 0; ph_launch[p] = 0; pending[p][0] = pending[p][1] = false; }
     }
 
@@ -550,7 +611,33 @@ struct ssasdpd_engine {
             sa = sdpd_args(step);
         }
         ph_begin(5);
-        sdpd_forces(st, sa);                       // substep 1
+        const bool nn_step = nn && nn_active && step > 0 && static_domain == 0;
+        if (nn_step) nn_ensure();
+        if (nn_step && step % nn_check_every != 0) {
+            // surrogate step: the pair kernel supplies Fbp, Frho and Q only,
+            // the network the accelerations
+            sa.skip_F = 1;
+            sdpd_forces(st, sa);
+            sa.skip_F = 0;
+            nn_predict(F);
+            nn_steps++;
+        } else {
+            sdpd_forces(st, sa);                   // substep 1
+            if (nn_rec && step > 0 && static_domain == 0 && step % nn_rec_every == 0) record_sample();
+            if (nn_step) {
+                // periodic check against the plain SDPD; fall back for the
+                // rest of the run when the relative error exceeds the budget
+                nn_predict(nn_F);
+                HIP_OK(hipMemsetAsync(nn_acc, 0, sizeof(double) * 2, st));
+                mpn_err(st, n, F, nn_F, solid, g, nn_acc);
+                double acc[2];
+                d2h(st, acc, nn_acc, 2);
+                HIP_OK(hipStreamSynchronize(st));
+                nn_last_err = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
+                nn_checks++;
+                if (!(nn_last_err <= nn_tol)) nn_active = false;
+            }
+        }
         ph_end(5);
         ph_begin(2);
         sdpd_correct(st, sa);                      // substep 2
@@ -1395,6 +1482,225 @@ int ssasdpd_ensemble_run(ssasdpd_ensemble *m, const uint64_t *seeds, uint32_t *t
 
 int ssasdpd_ensemble_stats(ssasdpd_ensemble *m, ssasdpd_stats *s) {
     *s = m->stats;
+    return 0;
+}
+
+struct ssasdpd_mpn {
+    ssa::Mpn *m;
+};
+
+int32_t ssasdpd_mpn_num_params(void) { return Mpn::P; }
+
+int ssasdpd_mpn_create(uint64_t seed, ssasdpd_mpn **out) {
+    GUARD_BEGIN
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(SSASDPD_ERR_HIP, "no HIP device");
+    auto *h = new ssasdpd_mpn{new Mpn(seed)};
+    *out = h;
+    return 0;
+    GUARD_END
+}
+
+void ssasdpd_mpn_destroy(ssasdpd_mpn *m) {
+    if (!m) return;
+    delete m->m;
+    delete m;
+}
+
+int ssasdpd_mpn_get_params(ssasdpd_mpn *m, float *w) {
+    GUARD_BEGIN
+    m->m->get_params(w);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_set_params(ssasdpd_mpn *m, const float *w) {
+    GUARD_BEGIN
+    m->m->set_params(w);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_set_scales(ssasdpd_mpn *m, const float *sc) {
+    m->m->rho0 = sc[0]; m->m->v_scale = sc[1]; m->m->rho_scale = sc[2]; m->m->nu_scale = sc[3];
+    m->m->acc_scale = sc[4];
+    return 0;
+}
+
+int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, int32_t n, int32_t W, float inv_h, const float *x, const float *X0,
+                           const int32_t *deg, const int32_t *nidx, const float *y, int32_t *index) {
+    GUARD_BEGIN
+    if (n <= 0 || W <= 0) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: empty graph");
+    for (int i = 0; i < n; i++) {
+        if (deg[i] < 0 || deg[i] > W) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: degree out of range");
+        for (int k = 0; k < deg[i]; k++) {
+            const int j = nidx[(size_t)k * n + i];
+            if (j < 0 || j >= n) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: neighbour index out of range");
+        }
+    }
+    ssa::Mpn *M = m->m;
+    MpnSample s;
+    s.n = n; s.W = W; s.inv_h = inv_h;
+    s.x = dalloc<float>((size_t)n * 3); s.X0 = dalloc<float>((size_t)n * 32);
+    s.deg = dalloc<int>(n); s.nidx = dalloc<int>((size_t)n * W); s.y = dalloc<float>((size_t)n * 3);
+    h2d(M->st, s.x, x, (size_t)n * 3); h2d(M->st, s.X0, X0, (size_t)n * 32); h2d(M->st, s.deg, deg, n);
+    h2d(M->st, s.nidx, nidx, (size_t)n * W); h2d(M->st, s.y, y, (size_t)n * 3);
+    HIP_OK(hipStreamSynchronize(M->st));
+    M->samples.push_back(s);
+    if (index) *index = (int32_t)M->samples.size() - 1;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_clear_samples(ssasdpd_mpn *m) {
+    for (auto &s : m->m->samples) { dfree(s.x); dfree(s.X0); dfree(s.deg); dfree(s.nidx); dfree(s.y); }
+    m->m->samples.clear();
+    return 0;
+}
+
+static MpnGraph sample_graph(const MpnSample &s) {
+    MpnGraph gr;
+    gr.n = s.n; gr.W = s.W; gr.inv_h = s.inv_h; gr.x = s.x; gr.deg = s.deg; gr.nidx = s.nidx;
+    return gr;
+}
+
+int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t idx, float *out) {
+    GUARD_BEGIN
+    ssa::Mpn *M = m->m;
+    if (idx < 0 || idx >= (int)M->samples.size()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no such sample");
+    const MpnSample &s = M->samples[idx];
+    M->forward(sample_graph(s), s.X0, nullptr);
+    d2h(M->st, out, M->out_buf(), (size_t)s.n * 3);
+    HIP_OK(hipStreamSynchronize(M->st));
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_grad(ssasdpd_mpn *m, int32_t idx, float *grad, double *loss) {
+    GUARD_BEGIN
+    ssa::Mpn *M = m->m;
+    if (idx < 0 || idx >= (int)M->samples.size()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no such sample");
+    const MpnSample &s = M->samples[idx];
+    float *dl = dalloc<float>(1);
+    HIP_OK(hipMemsetAsync(dl, 0, sizeof(float), M->st));
+    M->forward(sample_graph(s), s.X0, nullptr);
+    M->backward(sample_graph(s), s.X0, s.y, dl);
+    float l = 0.f;
+    d2h(M->st, &l, dl, 1);
+    if (grad) d2h(M->st, grad, M->g, Mpn::P);
+    HIP_OK(hipStreamSynchronize(M->st));
+    dfree(dl);
+    if (loss) *loss = l;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle_seed, double *mean_loss) {
+    GUARD_BEGIN
+    ssa::Mpn *M = m->m;
+    const int ns = (int)M->samples.size();
+    if (ns == 0) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no training samples");
+    float *dl = dalloc<float>(1);
+    std::vector<int> order(ns);
+    for (int i = 0; i < ns; i++) order[i] = i;
+    std::mt19937_64 rng(shuffle_seed);
+    double last = 0.0;
+    for (int ep = 0; ep < epochs; ep++) {
+        std::shuffle(order.begin(), order.end(), rng);
+        HIP_OK(hipMemsetAsync(dl, 0, sizeof(float), M->st));
+        for (int k = 0; k < ns; k++) {
+            const MpnSample &s = M->samples[order[k]];
+            M->forward(sample_graph(s), s.X0, nullptr);
+            M->backward(sample_graph(s), s.X0, s.y, dl);
+            M->adam(lr);
+        }
+        float l = 0.f;
+        d2h(M->st, &l, dl, 1);
+        HIP_OK(hipStreamSynchronize(M->st));
+        last = l / ns;
+    }
+    dfree(dl);
+    if (mean_loss) *mean_loss = last;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t *index) {
+    GUARD_BEGIN
+    if (!e->lists_built) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: engine has no neighbour lists yet");
+    ssa::Mpn *M = m->m;
+    const int n = e->n;
+    // the plain pair-force kernel on the current state is the target
+    std::vector<double> F(3 * (size_t)n);
+    int rc = ssasdpd_eval_forces(e, 0, F.data(), nullptr, nullptr);
+    if (rc) return rc;
+    e->nn_ensure();
+    mpn_features(e->st, n, e->x, e->v, e->vt, e->rho, e->nu, e->solid, M->rho0, M->v_scale, M->rho_scale,
+                 M->nu_scale, e->nn_X0, e->nn_xf);
+    std::vector<float> X0((size_t)n * 32), xf((size_t)n * 3), y((size_t)n * 3);
+    std::vector<int> dg(n), nx((size_t)n * e->W), sol(n);
+    d2h(e->st, X0.data(), e->nn_X0, X0.size());
+    d2h(e->st, xf.data(), e->nn_xf, xf.size());
+    d2h(e->st, dg.data(), e->deg, n);
+    d2h(e->st, nx.data(), e->nidx, nx.size());
+    d2h(e->st, sol.data(), e->solid, n);
+    HIP_OK(hipStreamSynchronize(e->st));
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++)
+            y[3 * (size_t)i + c] = sol[i] ? 0.f : (float)((F[3 * (size_t)i + c] - e->g[c]) / M->acc_scale);
+    return ssasdpd_mpn_add_sample(m, n, e->W, (float)(1.0 / e->h), xf.data(), X0.data(), dg.data(), nx.data(), y.data(),
+                                  index);
+    GUARD_END
+}
+
+int ssasdpd_mpn_time_forward(ssasdpd_mpn *m, int32_t idx, int32_t reps, double *ms_per_forward) {
+    GUARD_BEGIN
+    ssa::Mpn *M = m->m;
+    if (idx < 0 || idx >= (int)M->samples.size() || reps < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: bad sample/reps");
+    const MpnSample &s = M->samples[idx];
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    M->forward(sample_graph(s), s.X0, nullptr);
+    HIP_OK(hipEventRecord(a, M->st));
+    for (int r = 0; r < reps; r++) M->forward(sample_graph(s), s.X0, nullptr);
+    HIP_OK(hipEventRecord(b, M->st));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms_per_forward = ms / reps;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every) {
+    if (m && every < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn_record: every must be >= 1");
+    e->nn_rec = m ? m->m : nullptr;
+    e->nn_rec_every = every;
+    return 0;
+}
+
+int32_t ssasdpd_mpn_num_samples(ssasdpd_mpn *m) { return (int32_t)m->m->samples.size(); }
+
+int ssasdpd_set_surrogate(ssasdpd_engine *e, ssasdpd_mpn *m, int32_t check_every, double max_rel_err) {
+    if (m && check_every < 1) return set_error(SSASDPD_ERR_BAD_DESC, "surrogate: check_every must be >= 1");
+    e->nn = m ? m->m : nullptr;
+    e->nn_check_every = check_every;
+    e->nn_tol = max_rel_err;
+    e->nn_active = e->nn != nullptr;
+    e->nn_steps = e->nn_checks = 0;
+    e->nn_last_err = -1.0;
+    return 0;
+}
+
+int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *checks, double *last_err,
+                            int32_t *active) {
+    if (nn_steps) *nn_steps = e->nn_steps;
+    if (checks) *checks = e->nn_checks;
+    if (last_err) *last_err = e->nn_last_err;
+    if (active) *active = e->nn_active ? 1 : 0;
     return 0;
 }
 

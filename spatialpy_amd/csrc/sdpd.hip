@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
                 ft[p] = ((Real)1.0 / mi) * vsum * acc * dWdr / (r + (Real)0.001 * hR);
             }
             for (int d = 0; d < a.dim; d++) {
-                F[d] += (double)(fp * dx[d] + fv * dv[d] + ft[d]);
+                if (!a.skip_F) F[d] += (double)(fp * dx[d] + fv * dv[d] + ft[d]);
                 Fbp[d] += (double)(fbp * dx[d]);
             }
             Frho = Frho + (double)(rhoi * (mj / rhoj) * dv_dx * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr -

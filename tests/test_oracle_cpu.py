@@ -256,3 +256,35 @@ def test_bad_descriptor_is_rejected_before_gpu():
     sa.type[0] = 5      # out of 1..n_types
     with pytest.raises(SimulationError, match="BAD_DESC"):
         Engine(sa)
+
+
+def test_mpn_oracle_backward_matches_finite_differences():
+    """The surrogate's CPU oracle: its analytic gradient equals central
+    differences of its FP64 loss on a random graph (every parameter block
+    sampled)."""
+    from oracle import mpn_oracle as mo
+    x, X0, deg, nidx, inv_h, y = mo.random_graph(n=40, seed=2, h=0.3)
+    w = mo.init_params(5)
+    _, g = mo.loss_grad(w, x, X0, deg, nidx, inv_h, y)
+    rng = np.random.default_rng(1)
+    # a few indices from each block
+    sizes = [mo.NIN * mo.HID, mo.HID] + [int(np.prod(s)) for _, s in mo._L] * mo.LAYERS + [mo.HID * 3, 3]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    idx = []
+    for a, b in zip(offs[:-1], offs[1:]):
+        cand = [k for k in range(a, b) if abs(g[k]) > 1e-6] or [a]
+        idx += list(rng.choice(cand, size=min(2, len(cand)), replace=False))
+    for k in idx:
+        e = 1e-6 * max(1.0, abs(w[k]))
+        wp, wm = w.copy(), w.copy()
+        wp[k] += e
+        wm[k] -= e
+        fd = (mo.loss_grad(wp, x, X0, deg, nidx, inv_h, y)[0] - mo.loss_grad(wm, x, X0, deg, nidx, inv_h, y)[0]) / (2 * e)
+        assert abs(fd - g[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, g[k])
+
+
+def test_mpn_parameter_count_matches_library():
+    from oracle import mpn_oracle as mo
+    from spatialpy_amd import _abi
+    lib = _abi.load_library()
+    assert int(lib.ssasdpd_mpn_num_params()) == mo.P == 77571
