@@ -517,8 +517,9 @@ struct ssasdpd_engine {
         for (int attempt = 0; attempt < 4; attempt++) {
             HIP_OK(hipMemsetAsync(&derr->overflow, 0, sizeof(int), st));
             HIP_OK(hipMemsetAsync(&derr->max_deg, 0, sizeof(int), st));
+            // queries in particle order: each wave's list writes are coalesced
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
-                      deg, derr, sorted);
+                      deg, derr, nullptr);
             DevErr e = read_err();
             max_deg_seen = std::max(max_deg_seen, e.max_deg);
             if (!e.overflow) break;

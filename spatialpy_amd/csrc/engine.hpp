@@ -33,6 +33,19 @@ struct HipError {
 constexpr int WAVE = 64;
 constexpr int TREE_FAN = 64;
 
+// XCD-aware block order for particle-order kernels: the hardware deals
+// consecutive workgroups round-robin to the 8 XCDs, each with its own L2;
+// launching round_up(nb, 8) workgroups and mapping workgroup b to chunk
+// (b % 8) * ceil(nb/8) + b / 8 gives every XCD one contiguous range of
+// particles (neighbours in space share its L2).  Returns -1 past the end.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_chunk(int nb) {
+    const int per = (nb + N_XCD - 1) / N_XCD;
+    const int c = (int)(blockIdx.x % N_XCD) * per + (int)(blockIdx.x / N_XCD);
+    return c < nb ? c : -1;
+}
+inline dim3 xcd_grid(int nb) { return dim3((unsigned)(((nb + N_XCD - 1) / N_XCD) * N_XCD)); }
+
 // Device-side error record.
 struct DevErr {
     int code;          // ssasdpd_status

@@ -611,7 +611,9 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
             const int *__restrict__ cell_start, int n, int dim, double h, Grid g, KdView kd, int W,
             int *__restrict__ out_idx, double *__restrict__ out_r2, int *__restrict__ deg, DevErr *err,
             const int *__restrict__ order) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((m + 255) / 256);
+    if (chunk < 0) return;
+    int t = chunk * blockDim.x + threadIdx.x;
     if (t >= m) return;
     int qi = order ? order[t] : t;
     double qp[3] = {0.0, 0.0, 0.0};
@@ -679,8 +681,8 @@ void fr_search(hipStream_t s, const double *q, int m, const double *spts, const 
         v.parent = kd->parent; v.depth = kd->depth; v.cd = kd->cd; v.is_hi = kd->is_hi;
         v.leaf_of = kd->leaf_of; v.cv = kd->cv; v.valid = 1;
     }
-    hipLaunchKernelGGL(k_fr_search, dim3((m + 255) / 256), dim3(256), 0, s, q, m, spts, sorted, cell_start, n, dim,
-                       h, g, v, W, out_idx, out_r2, deg, err, order);
+    hipLaunchKernelGGL(k_fr_search, xcd_grid((m + 255) / 256), dim3(256), 0, s, q, m, spts, sorted, cell_start, n,
+                       dim, h, g, v, W, out_idx, out_r2, deg, err, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -688,7 +690,9 @@ void fr_search(hipStream_t s, const double *q, int m, const double *spts, const 
 __global__ void k_finish(int n, int W, const int *deg, const int *idx, const double *r2, double *r, double *Dij,
                          const double *mass, const double *rho, double h, DevErr *err, const int *type,
                          int *ntype) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int dg = deg[i];
     double mi = mass[i], ri = rho[i];
@@ -716,7 +720,7 @@ __global__ void k_finish(int n, int W, const int *deg, const int *idx, const dou
 void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2, double *r,
                   double *Dij, const double *mass, const double *rho, double h, DevErr *err, const int *type,
                   int *ntype) {
-    hipLaunchKernelGGL(k_finish, dim3((n + 255) / 256), dim3(256), 0, s, n, W, deg, idx, r2, r, Dij, mass, rho, h,
+    hipLaunchKernelGGL(k_finish, xcd_grid((n + 255) / 256), dim3(256), 0, s, n, W, deg, idx, r2, r, Dij, mass, rho, h,
                        err, type, ntype);
 }
 

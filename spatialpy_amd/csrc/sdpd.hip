@@ -46,7 +46,9 @@ __device__ __forceinline__ void apply_bcs(const SdpdArgs &a, int i) {
 __device__ __forceinline__ bool bad(double x) { return isnan(x) || !isfinite(x); }
 
 __global__ void k_predict(SdpdArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const size_t n = a.n;
     bool nan = bad(a.rho[i]);
@@ -98,7 +100,9 @@ __device__ __forceinline__ double chem_rhs(const SdpdArgs &a, int i, int r) {
 // static domains only Q is observable, so the momentum part is skipped.
 template <typename Real>
 __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const size_t n = a.n;
     const double h = a.h;
@@ -207,7 +211,9 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
 __device__ __forceinline__ double cube(double x) { return x * x * x; }
 
 __global__ void k_correct(SdpdArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n || a.static_domain) return;
     const size_t n = a.n;
     const bool filt = (a.step % 20) == 0;
@@ -237,7 +243,9 @@ __global__ void k_correct(SdpdArgs a) {
 }
 
 __global__ void k_bvf(SdpdArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const size_t n = a.n;
     if (a.solid[i] == 0 && a.static_domain == 0) {
@@ -278,7 +286,7 @@ __global__ void k_bvf(SdpdArgs a) {
     apply_bcs(a, i);
 }
 
-static dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
+static dim3 grid1(int n) { return xcd_grid((n + 255) / 256); }
 
 void sdpd_predict(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_predict, grid1(a.n), dim3(256), 0, s, a); }
 void sdpd_forces(hipStream_t s, const SdpdArgs &a) {

@@ -155,7 +155,9 @@ __device__ __forceinline__ double propensity(const Rxn &re, const uint32_t *xv, 
 // ---------------------------------------------------------------------------
 // Rates (simulate_rdme.cpp:113-152), one thread per voxel.
 __global__ void k_rates(RdmeArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int b = blockIdx.y;
     uint32_t *xx = a.xx + b * a.bs_xx;
@@ -178,7 +180,7 @@ __global__ void k_rates(RdmeArgs a) {
         double dd = 0.0;
         for (int k = 0; k < dg; k++) {
             size_t e = (size_t)k * a.n + i;
-            double dc = a.Dmat[(size_t)s * a.T + (a.type[a.nidx[e]] - 1)];
+            double dc = a.Dmat[(size_t)s * a.T + (a.ntype[e] - 1)];   // ntype[e] = type of neighbour e
             dd += dc * a.Dij[e];
         }
         if (b == 0) a.Ddiag[(size_t)i * S + s] = dd;
@@ -187,7 +189,7 @@ __global__ void k_rates(RdmeArgs a) {
     sdrate[i] = sd;
 }
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj) {
-    hipLaunchKernelGGL(k_rates, dim3((a.n + 255) / 256, n_traj), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_rates, dim3(xcd_grid((a.n + 255) / 256).x, n_traj), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
