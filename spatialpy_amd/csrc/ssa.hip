@@ -93,32 +93,49 @@ struct Rng {
 // host-computed exponentials in replay mode); a draw is one v_readlane pair.
 // Every word is consumed either as a uniform or as an exponential, in the
 // reference's draw order, so the values are those of Rng above.
-struct WaveRng {
-    int kind;
-    uint64_t seed, stream;
+// The stream's constant parameters, in LDS for the whole launch (read only
+// when a new batch of 64 words is drawn, so no registers are held for them).
+struct RngParams {
     const uint64_t *words;
     const double *expo;
     unsigned long long n_words;
+    uint64_t seed, stream;
+    int kind;
+};
+__shared__ RngParams g_rngp;
+
+// Lane i's uniform (x) and exponential (y) variates of word base + i.
+__device__ __noinline__ double2 rng_fill(unsigned long long base, int lane) {
+    const RngParams p = g_rngp;
+    const unsigned long long k = base + (unsigned long long)lane;
+    double2 r;
+    if (p.kind == 1) {
+        const bool ok = k < p.n_words;
+        r.x = ok ? word_uniform(p.words[k]) : 0.5;
+        r.y = ok ? p.expo[k] : 1.0;
+    } else {
+        const uint64_t w = philox64(p.seed, p.stream, k);
+        r.x = word_uniform(w);
+        r.y = word_expo(w);
+    }
+    return r;
+}
+
+struct WaveRng {
+    unsigned long long n_words;  // replay: words available (kind 1), else ~0
     unsigned long long pos;      // next word
     unsigned long long base;     // word held by lane 0
     double u_l, e_l;
     bool exhausted;
-    __device__ __forceinline__ void fill(int lane) {
+    __device__ __forceinline__ void refill(int lane) {
         base = pos & ~63ULL;
-        const unsigned long long k = base + (unsigned long long)lane;
-        if (kind == 1) {
-            const bool ok = k < n_words;
-            u_l = ok ? word_uniform(words[k]) : 0.5;
-            e_l = ok ? expo[k] : 1.0;
-        } else {
-            const uint64_t w = philox64(seed, stream, k);
-            u_l = word_uniform(w);
-            e_l = word_expo(w);
-        }
+        const double2 r = rng_fill(base, lane);
+        u_l = r.x;
+        e_l = r.y;
     }
-    __device__ __forceinline__ void init(int lane) { fill(lane); }
+    __device__ __forceinline__ void init(int lane) { refill(lane); }
     __device__ __forceinline__ int slot(int lane) {
-        if (pos - base >= 64) fill(lane);
+        if (pos - base >= 64) refill(lane);
         return (int)(pos - base);
     }
     __device__ __forceinline__ double pick(double x, int s) {
@@ -127,13 +144,13 @@ struct WaveRng {
         return __hiloint2double(hi, lo);
     }
     __device__ __forceinline__ double uniform(int lane) {
-        if (kind == 1 && pos >= n_words) { exhausted = true; return 0.5; }
+        if (pos >= n_words) { exhausted = true; return 0.5; }
         const int s = slot(lane);
         pos++;
         return pick(u_l, s);
     }
     __device__ __forceinline__ double exponential(int lane) {
-        if (kind == 1 && pos >= n_words) { exhausted = true; return 1.0; }
+        if (pos >= n_words) { exhausted = true; return 1.0; }
         const int s = slot(lane);
         pos++;
         return pick(e_l, s);
@@ -721,9 +738,15 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         }
     }
 
+    if (lane == 0) {
+        g_rngp.words = a.words; g_rngp.expo = a.expo; g_rngp.n_words = a.n_words;
+        g_rngp.seed = seed_; g_rngp.stream = a.stream_id; g_rngp.kind = a.rng_kind;
+    }
+    __syncthreads();
     WaveRng rng;
-    rng.kind = a.rng_kind; rng.seed = seed_; rng.stream = a.stream_id; rng.words = a.words; rng.expo = a.expo;
-    rng.n_words = a.n_words; rng.pos = q.rng_pos; rng.exhausted = false;
+    rng.n_words = a.rng_kind == 1 ? a.n_words : ~0ULL;
+    rng.pos = q.rng_pos;
+    rng.exhausted = false;
     rng.init(lane);
     const bool textbook = a.textbook != 0;
     const bool lead = lane == 0;
