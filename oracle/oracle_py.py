@@ -193,7 +193,8 @@ class Oracle:
         S = self.dh.desc.n_species
         Sc = self.dh.desc.n_chem_species
         res = dict(xx=np.zeros((k, n, S), np.uint32), C=np.zeros((k, n, Sc)), x=np.zeros((k, n, 3)),
-                   v=np.zeros((k, n, 3)), rho=np.zeros((k, n)), steps=np.zeros(k, np.uint32))
+                   v=np.zeros((k, n, 3)), rho=np.zeros((k, n)), bvf_phi=np.zeros((k, n)), nu=np.zeros((k, n)),
+                   steps=np.zeros(k, np.uint32))
         o = _abi.Outputs()
         o.n_snapshots = k
         o.xx = res["xx"].ctypes.data_as(C.POINTER(C.c_uint32))
@@ -201,6 +202,8 @@ class Oracle:
         o.x = res["x"].ctypes.data_as(C.POINTER(C.c_double))
         o.v = res["v"].ctypes.data_as(C.POINTER(C.c_double))
         o.rho = res["rho"].ctypes.data_as(C.POINTER(C.c_double))
+        o.bvf_phi = res["bvf_phi"].ctypes.data_as(C.POINTER(C.c_double))
+        o.nu = res["nu"].ctypes.data_as(C.POINTER(C.c_double))
         o.steps = res["steps"].ctypes.data_as(C.POINTER(C.c_uint32))
         self._keep = (words, expo)
         rc = lib().oracle_run(self.h, C.byref(self._rng(seed, words, expo)), C.byref(o))

@@ -43,6 +43,7 @@
 #include "propensities.hpp"
 #include "simulate.hpp"
 #include "simulate_rdme.hpp"
+#include "output.hpp"
 
 int debug_flag;
 std::mt19937_64 rng;
@@ -255,7 +256,13 @@ int main(int argc, char **argv) {
     std::vector<uint32_t> snap_xx;
     std::vector<double> snap_x, snap_v, snap_rho, snap_C;
     std::vector<uint32_t> snap_steps;
+    const bool write_vtk = g_meta.count("write_vtk") && geti("write_vtk");
     auto snapshot = [&](unsigned step) {
+        if (write_vtk) {
+            /* the reference's own VTK writer, synchronously, into the cwd */
+            Spatialpy::output_vtk__sync_step(system, (int)step);
+            Spatialpy::output_vtk__async_step(system);
+        }
         for (auto &p : system->particles) {
             for (int s = 0; s < S; s++) snap_xx.push_back(p.xx[s]);
             for (int k = 0; k < 3; k++) { snap_x.push_back(p.x[k]); snap_v.push_back(p.v[k]); }

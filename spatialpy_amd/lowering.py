@@ -231,7 +231,7 @@ def output_schedule(output_steps, nt):
 
 
 def write_harness_dir(sa: SystemArrays, path, seed, run_steps=None, debug_flag=0,
-                      force_step=0, nbr_step=0, threads=0):
+                      force_step=0, nbr_step=0, threads=0, write_vtk=0):
     """Write a model in the raw format read by oracle/ref_harness.cpp."""
     sa.normalized()
     os.makedirs(path, exist_ok=True)
@@ -251,7 +251,7 @@ def write_harness_dir(sa: SystemArrays, path, seed, run_steps=None, debug_flag=0
         "g0": repr(float(sa.gravity[0])), "g1": repr(float(sa.gravity[1])),
         "g2": repr(float(sa.gravity[2])), "n_bcs": len(sa.bcs),
         "n_output_steps": int(sa.output_steps.size), "force_step": force_step,
-        "nbr_step": nbr_step, "threads": threads,
+        "nbr_step": nbr_step, "threads": threads, "write_vtk": int(write_vtk),
     }
     if run_steps is not None:
         meta["run_steps"] = run_steps

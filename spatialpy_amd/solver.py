@@ -70,6 +70,17 @@ class Result:
             raise SimulationError(f"unknown property {property_name}")
         return out[0] if out.shape[0] == 1 else out
 
+    def export_vtk(self, folder):
+        """Write this trajectory's snapshots as the reference engine does:
+        output{k}.vtk and output0_boundingBox.vtk in folder (the files the
+        reference's Result reads, core/result.py:173-183)."""
+        from .lowering import output_schedule
+        from .vtk import write_result
+        names = list(self.model.listOfSpecies)
+        steps = output_schedule(self.sa.output_steps, self.sa.nt)
+        write_result(folder, self.data, self.sa, names, steps)
+        return folder
+
     def __eq__(self, other):
         if not isinstance(other, Result):
             return False

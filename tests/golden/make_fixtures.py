@@ -66,6 +66,34 @@ def make_case(name, spec):
     print(name, "n =", sa.n, "events =", out["counters"].tolist())
 
 
+VTK_CASES = {
+    "vtk_c1_birth_death": dict(gen="birth_death_2d", params=dict(seed=1, t_end=2, num_points=3), seed=7),
+    "vtk_c3_dam_break_tiny": dict(gen="dam_break_2d",
+                                  params=dict(nx=6, ny=6, spacing=0.08, n_mol=20, nt=4, out_every=2, dt=2e-4),
+                                  seed=5),
+}
+
+
+def make_vtk_case(name, spec):
+    """The reference engine's own VTK files (output_vtk__sync_step +
+    output_vtk__async_step, run in the harness's working directory)."""
+    sa = getattr(workloads, spec["gen"])(**spec["params"])
+    dst = os.path.join(HERE, name)
+    os.makedirs(dst, exist_ok=True)
+    with tempfile.TemporaryDirectory() as d:
+        lowering.write_harness_dir(sa, d, seed=spec["seed"], write_vtk=1)
+        r = subprocess.run([HARNESS, d], capture_output=True, text=True, cwd=d)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: harness failed: {r.stdout[-500:]} {r.stderr[-500:]}")
+        files = sorted(f for f in os.listdir(d) if f.endswith(".vtk"))
+        for f in files:
+            with open(os.path.join(d, f)) as a, open(os.path.join(dst, f), "w") as b:
+                b.write(a.read())
+    with open(os.path.join(dst, "case.json"), "w") as f:
+        json.dump(dict(gen=spec["gen"], params=spec["params"], seed=spec["seed"], files=files), f)
+    print(name, "n =", sa.n, "files", files)
+
+
 def make_ann_test1():
     """ANN's own test 1: 20 2-D points, bucket 1, sliding midpoint ('suggest');
     test1.save records the tree's node counts and depth."""
@@ -81,6 +109,12 @@ def make_ann_test1():
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
+    only = sys.argv[1:]
     for name, spec in CASES.items():
-        make_case(name, spec)
-    make_ann_test1()
+        if not only or name in only:
+            make_case(name, spec)
+    for name, spec in VTK_CASES.items():
+        if not only or name in only:
+            make_vtk_case(name, spec)
+    if not only:
+        make_ann_test1()
