@@ -26,6 +26,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per launch of k_ssa and per step of the particle path, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (profiles/cylinder_1M_traffic.txt; None when it is absent)."""
+    path = os.path.join(ROOT, "profiles", "cylinder_1M_traffic.txt")
+    out = {"k_ssa": None, "particle_path": None}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        if line.startswith("k_ssa traffic_bytes_per_launch="):
+            out["k_ssa"] = float(line.split("=")[1])
+        if line.startswith("particle_path traffic_bytes_per_step="):
+            out["particle_path"] = float(line.split("=")[1])
+    return out
+
+
 def algorithmic_bytes_per_particle_step(dim, S, E):
     """Compulsory HBM traffic per particle-step of the SDPD + rate path:
     read x, v, vt (3*8d), rho, mass, nu (24), type (4); write x, v, vt (3*8d),
@@ -166,8 +182,9 @@ def main():
     B = algorithmic_bytes_per_particle_step(3, sa.n_species, E)
     hbm_ms = dph["search"][0] + dph["sdpd"][0] + dph["forces"][0] + dph["rates_queue"][0] + dph["bin_tree"][0]
     achieved = B * n * args.steps / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
+    traffic = measured_traffic()
     particle_roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic["particle_path"],
                      "kernels": "binning + fixed-radius search + D_ij + SDPD substeps + rates/queue build",
                      "algorithmic_bytes_per_particle_step": B, "ms_per_step": round(hbm_ms / args.steps, 4)}
     # the SSA event loop: one dependent chain per trajectory; its algorithmic
@@ -181,7 +198,7 @@ def main():
     ssa_bytes = b_react * n_re + b_diff * n_di
     ssa_gbs = ssa_bytes / (ssa_ms * 1e-3) / 1e9 if ssa_ms > 0 else 0.0
     ssa_roof = {"bound": "hbm", "achieved": round(ssa_gbs, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ssa_gbs / HBM_PEAK_GBS, 7), "traffic": None,
+                "frac": round(ssa_gbs / HBM_PEAK_GBS, 7), "traffic": traffic["k_ssa"],
                 "kernel": "k_ssa (event loop; latency-bound, one wavefront per trajectory)",
                 "algorithmic_bytes_per_event": {"reaction": b_react, "diffusion": b_diff},
                 "events": int(n_re + n_di), "us_per_event": round(1e3 * ssa_ms / max(n_re + n_di, 1), 3),
