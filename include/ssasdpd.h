@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define SSASDPD_ABI_VERSION 2
+#define SSASDPD_ABI_VERSION 3
 
 /* Status codes.  The reference prints and calls exit(1) for each failure
  * class (particle.cpp:88-126, 189-206; simulate_rdme.cpp:262-294, 322-349,
@@ -51,7 +51,15 @@ enum ssasdpd_status {
  * X, Y are the species counts read as signed int (the generated functions
  * take `const int *x`, solver.py:345).  The deterministic (ODE) right-hand
  * side of the same reaction drops `vol` and the -1:  0: k  1: k*X
- * 2: ((k*X)*Y)  3: ((k*X)*X)  (reaction.py:297-311, solver.py:160-190). */
+ * 2: ((k*X)*Y)  3: ((k*X)*X)  (reaction.py:297-311, solver.py:160-190).
+ * Kind 4 is any other propensity (custom propensity functions, reaction.py:
+ * 120-135, 325-330): a = start of its propensity program in the
+ * descriptor's `code`, b = start of its deterministic-rate program, k is
+ * unused.  Programs are the expression VM's bytecode (spatialpy_amd/
+ * expression.py compiles the C++ text solver.py:160-190, 344-373
+ * generates), evaluated with the arguments of the generated functions:
+ * counts (as int) or concentrations, t, vol, data_fn[], and type (for
+ * type_mask, the generated `if (sd == type_i || ...)`). */
 typedef struct ssasdpd_reaction {
     int32_t kind;
     int32_t a;            /* first reactant species index (kinds 1-3) */
@@ -136,6 +144,15 @@ typedef struct ssasdpd_desc {
 
     const ssasdpd_bc *bcs;    /* [n_bcs]                                        */
     const uint32_t *output_steps; /* [n_output_steps] (core/timespan.py:72-85)  */
+
+    /* expression-VM programs of kind-4 reactions (may be empty)            */
+    int32_t n_code, n_consts;
+    const uint32_t *code;     /* [n_code] instruction words                     */
+    const double *consts;     /* [n_consts] constant pool                        */
+    /* data functions (solver.py:192-198, 239-249): value of function k at
+     * particle i's initial position, read by programs as data_fn[k]          */
+    int32_t n_data_fn, pad1;
+    const double *data_fn;    /* [n][n_data_fn]                                 */
 } ssasdpd_desc;
 
 /* Output snapshot buffers, caller-allocated with ssasdpd_num_snapshots()
@@ -224,6 +241,15 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
  * from at least one step; static domains return gravity only (the reference
  * computes no momentum there that any later step reads). */
 int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp, double *Frho);
+
+/* Evaluate one expression-VM program on the GPU for m argument sets (the
+ * device interpreter the engine uses): x [m][n_x] (species as int when
+ * x_is_double is 0 -- values must be integral -- else concentrations), vol,
+ * t [m], df [m][n_df] (may be NULL when n_df is 0), out [m].  For testing
+ * compiled propensities against the generated C++. */
+int ssasdpd_vm_eval(const uint32_t *code, int32_t n_code, const double *consts, int32_t n_consts, int32_t start,
+                    int32_t x_is_double, int32_t n_x, int32_t n_df, int32_t m, const double *x, const double *vol,
+                    const double *t, const double *df, double *out);
 
 /* ---- Ensembles of different models ------------------------------------
  * n_models static-domain models (uneven domain sizes, each its own

@@ -337,6 +337,10 @@ struct Sim {
     int gauss_seidel = 1;
     std::vector<ssasdpd_reaction> rxn;
     std::vector<int> jcN, irN, prN, jcG, irG, Ndense;
+    /* custom propensities: expression-VM programs, data functions [n][ndf] */
+    std::vector<uint32_t> code;
+    std::vector<double> consts, dfn;
+    int ndf = 0;
     std::vector<double> Dmat;
     std::vector<ssasdpd_bc> bcs;
     std::vector<uint32_t> out_steps;
@@ -480,11 +484,79 @@ struct Sim {
         old_rho[i] = rho[i];
     }
 
+    /* The expression VM's bytecode (spatialpy_amd/expression.py) evaluated
+     * in C semantics: the generated functions' `int` arithmetic (32-bit
+     * wrap, truncating division) and IEEE double arithmetic, this host's
+     * libm for the math functions.  xi: counts (a propensity, const int *x)
+     * or xd: concentrations (a deterministic rate, const double *x). */
+    const double *df_of(int i) const { return ndf ? &dfn[(size_t)i * ndf] : nullptr; }
+    static double vm_i(long long v) { return (double)(int32_t)(uint32_t)(uint64_t)v; }
+    double vm(int pc, const int *xi, const double *xd, double vol, double t, const double *df) const {
+        double st[16];
+        int sp = 0;
+        for (;;) {
+            const uint32_t w = code[pc++];
+            const int op = w & 63, src = (w >> 6) & 7, rev = (w >> 9) & 1, idx = (int)(w >> 10);
+            if (op == 0) return st[sp - 1];
+            if (op == 10) { st[sp - 1] = -st[sp - 1]; continue; }
+            if (op == 11) { st[sp - 1] = vm_i(-(long long)(int)st[sp - 1]); continue; }
+            if (op == 20) {
+                static double (*const f1[])(double) = {exp, log, log10, log2, log1p, expm1, sqrt, sin, cos, tan,
+                                                       asin, acos, atan, sinh, cosh, tanh, asinh, acosh, atanh,
+                                                       erf, erfc, lgamma, fabs, floor, ceil, trunc};
+                st[sp - 1] = f1[idx](st[sp - 1]);
+                continue;
+            }
+            double b;
+            switch (src) {
+            case 1: b = consts[idx]; break;
+            case 2: b = xi ? (double)xi[idx] : xd[idx]; break;
+            case 3: b = vol; break;
+            case 4: b = t; break;
+            case 5: b = df[idx]; break;
+            default: b = st[--sp]; break;
+            }
+            if (op == 1) { st[sp++] = b; continue; }
+            double l = st[sp - 1], r = b;
+            if (rev) { l = b; r = st[sp - 1]; }
+            double y;
+            switch (op) {
+            case 2: y = l + r; break;
+            case 3: y = l - r; break;
+            case 4: y = l * r; break;
+            case 5: y = l / r; break;
+            case 6: y = vm_i((long long)(int)l + (int)r); break;
+            case 7: y = vm_i((long long)(int)l - (int)r); break;
+            case 8: y = vm_i((long long)(int)l * (int)r); break;
+            case 9: {
+                const int a = (int)l, c = (int)r;
+                y = c == 0 ? 0.0 : (a == INT32_MIN && c == -1) ? (double)INT32_MIN : (double)(a / c);
+                break;
+            }
+            case 12: y = l < r; break;
+            case 13: y = l <= r; break;
+            case 14: y = l > r; break;
+            case 15: y = l >= r; break;
+            case 16: y = l == r; break;
+            case 17: y = l != r; break;
+            case 18: y = (l != 0.0 && r != 0.0); break;
+            case 19: y = (l != 0.0 || r != 0.0); break;
+            default: {
+                static double (*const f2[])(double, double) = {pow, atan2, hypot, fmod, copysign, remainder};
+                y = f2[idx - 32](l, r);
+            }
+            }
+            st[sp - 1] = y;
+        }
+    }
+
     double chem_rhs(int i, int r) const {
-        /* generated det<name> functions (solver.py:160-190) */
+        /* generated det<name> functions (solver.py:160-190), called with
+         * vol = mass/rho and t = current_step*dt (model.cpp:181-184) */
         const ssasdpd_reaction &re = rxn[r];
         if (re.type_mask && !((re.type_mask >> type[i]) & 1ULL)) return 0.0;
         const double *c = &C[(size_t)i * Sc];
+        if (re.kind == 4) return vm(re.b, nullptr, c, mass[i] / rho[i], current_step * dt, df_of(i));
         switch (re.kind) {
         case 0: return re.k;
         case 1: return re.k * c[re.a];
@@ -623,12 +695,13 @@ struct Sim {
     }
 
     /* ---------------- RDME ---------------- */
-    double propensity(int r, const uint32_t *xv, double vol, int tp) const {
+    double propensity(int r, const uint32_t *xv, double vol, int tp, double t, const double *df) const {
         /* generated propensity functions (solver.py:344-373,
          * spatialpy/core/reaction.py:260-313) */
         const ssasdpd_reaction &re = rxn[r];
         if (re.type_mask && !((re.type_mask >> tp) & 1ULL)) return 0.0;
         const int *xs = (const int *)xv;
+        if (re.kind == 4) return vm(re.a, xs, nullptr, vol, t, df);
         switch (re.kind) {
         case 0: return re.k * vol;
         case 1: return re.k * xs[re.a];
@@ -642,7 +715,7 @@ struct Sim {
             srrate[i] = 0.0;
             for (int j = 0; j < R; j++) {
                 double vol = mass[i] / rho[i];
-                rrate[(size_t)i * R + j] = propensity(j, &xx[(size_t)i * S], vol, type[i]);
+                rrate[(size_t)i * R + j] = propensity(j, &xx[(size_t)i * S], vol, type[i], 0.0, df_of(i));
                 srrate[i] += rrate[(size_t)i * R + j];
             }
         }
@@ -842,7 +915,7 @@ struct Sim {
                 if (re >= R) {
                     if (cum != srrate[sv]) {
                         double rd = 0.0;
-                        for (int j = 0; j < R; j++) rd += (rr[j] = propensity(j, xs, vol, type[sv]));
+                        for (int j = 0; j < R; j++) rd += (rr[j] = propensity(j, xs, vol, type[sv], tt, df_of(sv)));
                         srrate[sv] = rd;
                     }
                     if (srrate[sv] == 0.0) continue;
@@ -858,7 +931,7 @@ struct Sim {
                 for (int k = jcG[S + re]; k < jcG[S + re + 1]; k++) {
                     int j = irG[k];
                     double old = rr[j];
-                    rd += (rr[j] = propensity(j, xs, vol, type[sv])) - old;
+                    rd += (rr[j] = propensity(j, xs, vol, type[sv], tt, df_of(sv))) - old;
                 }
                 srrate[sv] += rd;
                 total_reactions++;
@@ -917,9 +990,9 @@ struct Sim {
                     for (int k = jcG[spec]; k < jcG[spec + 1]; k++) {
                         int j = irG[k];
                         double old = rr[j];
-                        rd += (rr[j] = propensity(j, xs, vol, type[sv])) - old;
+                        rd += (rr[j] = propensity(j, xs, vol, type[sv], tt, df_of(sv))) - old;
                         old = rrdst[j];
-                        rrd += (rrdst[j] = propensity(j, xd, vol, type[dest])) - old;
+                        rrd += (rrdst[j] = propensity(j, xd, vol, type[dest], tt, df_of(dest))) - old;
                     }
                     srrate[sv] += rd;
                     srrate[dest] += rrd;
@@ -1063,6 +1136,14 @@ void *oracle_create(const ssasdpd_desc *d, int gauss_seidel) {
         s->irG.assign(d->irG, d->irG + s->jcG.back());
     } else if (d->n_chem_species > 0 && d->diffusion_matrix) {
         s->Dmat.assign(d->diffusion_matrix, d->diffusion_matrix + (size_t)d->n_chem_species * T);
+    }
+    if (d->n_code > 0) {
+        s->code.assign(d->code, d->code + d->n_code);
+        s->consts.assign(d->consts, d->consts + d->n_consts);
+    }
+    if (d->n_data_fn > 0) {
+        s->ndf = d->n_data_fn;
+        s->dfn.assign(d->data_fn, d->data_fn + (size_t)n * d->n_data_fn);
     }
     if (R > 0) {
         s->rxn.assign(d->reactions, d->reactions + R);

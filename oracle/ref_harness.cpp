@@ -21,6 +21,12 @@
  * force substep, snapshots) and events.txt parsed from the engine's own
  * debug-level-1 event lines (simulate_rdme.cpp:282-283, 405-413).
  *
+ * Models with custom propensities: meta key prop_lib names a shared library
+ * built from the C++ text the reference generator writes for each reaction
+ * (solver.py:160-190, 344-373; prop_<r> and det_<r>), whose functions
+ * replace the table's; n_data_fn > 0 reads data_fn.f64 in the generated
+ * input_data_fn layout [function][particle] (solver.py:192-198, 239-249).
+ *
  * Deliberate differences from a generated program: Particle::F, Fbp, Frho
  * and vt (left uninitialised by Particle::Particle, particle.cpp:70-83) are
  * set to 0 after the particles are created; N_dense is padded with zeros so
@@ -33,6 +39,7 @@
 #include <cstring>
 #include <map>
 #include <ctime>
+#include <dlfcn.h>
 #include <unistd.h>
 #include <pthread.h>
 #include <random>
@@ -114,6 +121,19 @@ template <int r> double chem_fn(double *x, double t, double vol, double *data_fn
 Spatialpy::PropensityFun g_prop_tab[16] = {RX16((Spatialpy::PropensityFun)prop_fn)};
 Spatialpy::ChemRxnFun g_chem_tab[16] = {RX16(chem_fn)};
 
+/* generated functions from the model's shared library, when given */
+void load_prop_lib(const std::string &path, int R) {
+    void *h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (!h) { fprintf(stderr, "dlopen %s: %s\n", path.c_str(), dlerror()); exit(2); }
+    for (int r = 0; r < R; r++) {
+        char nm[32];
+        snprintf(nm, sizeof nm, "prop_%d", r);
+        if (void *f = dlsym(h, nm)) g_prop_tab[r] = (Spatialpy::PropensityFun)f;
+        snprintf(nm, sizeof nm, "det_%d", r);
+        if (void *f = dlsym(h, nm)) g_chem_tab[r] = (Spatialpy::ChemRxnFun)f;
+    }
+}
+
 }  // namespace
 
 namespace Spatialpy {
@@ -190,6 +210,9 @@ int main(int argc, char **argv) {
     auto rxd = load<double>("rx_k.f64", Rmax);
     auto rxm = load<uint64_t>("rx_mask.u64", Rmax);
     for (int r = 0; r < Rmax; r++) g_rx.push_back({rxk[3 * r], rxk[3 * r + 1], rxk[3 * r + 2], rxd[r], rxm[r]});
+    if (g_meta.count("prop_lib")) load_prop_lib(g_dir + "/" + g_meta.at("prop_lib"), Rmax);
+    const int ndf = g_meta.count("n_data_fn") ? geti("n_data_fn") : 0;
+    auto dfv = load<double>("data_fn.f64", (size_t)ndf * n);
     std::vector<int32_t> jcN32 = load<int32_t>("jcN.i32", R + 1 > 1 ? R + 1 : 0);
     size_t nnzN = R > 0 ? (size_t)jcN32[R] : 0;
     auto irN32 = load<int32_t>("irN.i32", nnzN);
@@ -220,7 +243,7 @@ int main(int argc, char **argv) {
     static const char *names[] = {"s0", "s1", "s2", "s3", "s4", "s5", "s6", "s7", 0};
 
     /* __SYSTEM_CONFIG__ (solver.py:375-419) */
-    ParticleSystem *system = new ParticleSystem(T, Sc, Rc, S, R, 0);
+    ParticleSystem *system = new ParticleSystem(T, Sc, Rc, S, R, ndf);
     system->static_domain = geti("static_domain");
     system->subdomain_diffusion_matrix = Dmat.data();
     system->stoichiometric_matrix = Ndense.data();
@@ -247,6 +270,8 @@ int main(int argc, char **argv) {
                                                 mass[i], 0.0, rho[i], fixed[i]));
         Particle *p = &system->particles.back();
         for (int s = 0; s < Sc; s++) p->C[s] = (double)u0[(size_t)i * Sc + s];
+        if (S > 0)   /* __DATA_FUNCTION_ASSIGN__ (solver.py:192-198) */
+            for (int k = 0; k < ndf; k++) p->data_fn[k] = dfv[(size_t)k * n + i];
         for (int k = 0; k < 3; k++) { p->v[k] = vin[3 * i + k]; p->vt[k] = 0; p->F[k] = 0; p->Fbp[k] = 0; }
         p->Frho = 0;
         p->bvf_phi = 0;

@@ -6,7 +6,8 @@ engine as hand-written HIP kernels in libssasdpd.so, called through a C ABI
 """
 from .core.errors import ModelError, DomainError, SimulationError, SimulationTimeout
 from .core.model import (Model, Species, Parameter, Reaction, TimeSpan, PlaceInitialCondition,
-                         UniformInitialCondition, ScatterInitialCondition, BoundaryCondition)
+                         UniformInitialCondition, ScatterInitialCondition, BoundaryCondition,
+                         DataFunction)
 from .core.domain import Domain
 from .lowering import SystemArrays, output_schedule
 from .solver import Solver, Result

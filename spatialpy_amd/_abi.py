@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 STATUS_NAMES = {
@@ -57,6 +57,8 @@ class Desc(C.Structure):
         ("diffusion_matrix", _dp), ("reactions", C.POINTER(Reaction)),
         ("jcN", _ip), ("irN", _ip), ("prN", _ip), ("jcG", _ip), ("irG", _ip), ("N_dense", _ip),
         ("bcs", C.POINTER(BC)), ("output_steps", _up),
+        ("n_code", C.c_int32), ("n_consts", C.c_int32), ("code", _up), ("consts", _dp),
+        ("n_data_fn", C.c_int32), ("pad1", C.c_int32), ("data_fn", _dp),
     ]
 
 
@@ -131,6 +133,8 @@ def _declare(lib):
     lib.ssasdpd_mt19937_64_stream.argtypes = [C.c_uint64, C.c_uint64, _u64p, _dp]
     lib.ssasdpd_phase_times.argtypes = [P, _dp, _u64p, C.c_int32]
     lib.ssasdpd_eval_forces.argtypes = [P, C.c_int32, _dp, _dp, _dp]
+    lib.ssasdpd_vm_eval.argtypes = [_up, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_int32, _dp, _dp, _dp, _dp, _dp]
     lib.ssasdpd_ensemble_create.argtypes = [C.POINTER(C.POINTER(Desc)), C.c_int32, C.POINTER(P)]
     lib.ssasdpd_ensemble_destroy.argtypes = [P]
     lib.ssasdpd_ensemble_destroy.restype = None
@@ -169,7 +173,7 @@ def _declare(lib):
                  "ssasdpd_mpn_set_params", "ssasdpd_mpn_set_scales", "ssasdpd_mpn_add_sample",
                  "ssasdpd_mpn_add_engine_sample", "ssasdpd_mpn_clear_samples", "ssasdpd_mpn_forward",
                  "ssasdpd_mpn_grad", "ssasdpd_mpn_train", "ssasdpd_set_surrogate", "ssasdpd_surrogate_stats",
-                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward"]:
+                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward", "ssasdpd_vm_eval"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

@@ -51,43 +51,74 @@ class Parameter:
 
 
 class Reaction:
-    """spatialpy/core/reaction.py.  Mass-action reactions only (rate given);
-    the propensity expression text follows Reaction._create_mass_action
-    (reaction.py:260-313)."""
+    """spatialpy/core/reaction.py:73-140.  A mass-action reaction (rate: a
+    Parameter, its name or a number) or a custom one (propensity_function
+    and/or ode_propensity_function, each defaulting to the other).  Both
+    strings are kept as the reference keeps them, after
+    Reaction._create_custom_propensity (reaction.py:260-330); the solver
+    turns them into C++ text and VM programs (spatialpy_amd/expression.py)."""
 
     def __init__(self, name=None, reactants=None, products=None, propensity_function=None,
                  ode_propensity_function=None, rate=None, annotation=None, restrict_to=None):
+        from ..expression import ExpressionError, mass_action_texts, reaction_text
         self.name = name
-        self.reactants = OrderedDict(reactants or {})
-        self.products = OrderedDict(products or {})
+        self.reactants = OrderedDict()
+        for s, c in (reactants or {}).items():
+            key = s.name if isinstance(s, Species) else s
+            self.reactants[key] = self.reactants.get(key, 0) + c
+        self.products = OrderedDict()
+        for s, c in (products or {}).items():
+            key = s.name if isinstance(s, Species) else s
+            self.products[key] = self.products.get(key, 0) + c
+        if isinstance(rate, Parameter):
+            rate = rate.name
+        elif isinstance(rate, (int, float)) and not isinstance(rate, bool):
+            rate = str(rate)
         self.marate = rate
         self.annotation = annotation
-        self.massaction = propensity_function is None
-        self.propensity_function = propensity_function
-        self.ode_propensity_function = ode_propensity_function
         if restrict_to is not None and not isinstance(restrict_to, list):
             restrict_to = [restrict_to]
         self.restrict_to = restrict_to
-        if self.massaction and rate is None:
-            raise ModelError(f"Reaction {name}: give either rate (mass action) or propensity_function")
-        order = sum(self.reactants.values())
-        if self.massaction and order > 2:
-            raise ModelError("A mass-action reaction cannot involve more than two of one species "
-                             "or one of two species")
+        if rate is not None:
+            self.massaction = True
+            self.type = "mass-action"
+            try:
+                prop, ode = mass_action_texts(rate, self.reactants)
+            except ExpressionError as e:
+                raise ModelError(str(e)) from e
+        else:
+            self.massaction = False
+            self.type = "customized"
+            if propensity_function is None and ode_propensity_function is None:
+                raise ModelError(f"Reaction {name}: give either rate (mass action) or propensity_function")
+            prop = propensity_function if propensity_function is not None else ode_propensity_function
+            ode = ode_propensity_function if ode_propensity_function is not None else propensity_function
+        try:
+            self.propensity_function = reaction_text(prop)
+            self.ode_propensity_function = reaction_text(ode)
+        except (SyntaxError, ExpressionError) as e:
+            raise ModelError(f"Reaction {name}: invalid propensity function: {e}") from e
 
     def propensity_expression(self):
-        """The C expression SpatialPy generates for this mass-action
-        reaction (reaction.py:260-313 through the expression converter)."""
-        k = self.marate.name if isinstance(self.marate, Parameter) else str(self.marate)
-        names = [(s.name if isinstance(s, Species) else s, c) for s, c in self.reactants.items()]
-        if not names:
-            return f"({k}*vol)"
-        if len(names) == 1 and names[0][1] == 2:
-            x = names[0][0]
-            return f"((({k}*{x})*({x}-1))/vol)"
-        if len(names) == 1:
-            return f"({k}*{names[0][0]})"
-        return f"((({k}*{names[0][0]})*{names[1][0]})/vol)"
+        """The propensity string as the reference keeps it (stage A of the
+        expression pipeline, core/reaction.py:325-330)."""
+        return self.propensity_function
+
+
+class DataFunction:
+    """spatialpy/core/datafunction.py: a named function of the initial
+    particle position, subclassed with map(point) -> float.  Its values
+    are taken once at t = 0 and read by propensities as the data function's
+    name."""
+
+    def __init__(self, name=None):
+        if name is not None:
+            self.name = name
+        if getattr(self, "name", None) is None:
+            raise ModelError("DataFunction must have a 'name'")
+
+    def map(self, point):
+        raise ModelError(f"{self.name}: DataFunction.map() must be implemented.")
 
 
 class TimeSpan:
@@ -245,8 +276,14 @@ class Model:
     def add_boundary_condition(self, bc):
         self.listOfBoundaryConditions.append(bc)
 
-    def add_data_function(self, df):
-        raise ModelError("DataFunctions are not supported by this engine yet")
+    def add_data_function(self, data_function):
+        """core/model.py add_data_function: one DataFunction or a list."""
+        for df in data_function if isinstance(data_function, list) else [data_function]:
+            if not isinstance(df, DataFunction) and type(df).__name__ != "DataFunction" and \
+                    not hasattr(df, "map"):
+                raise ModelError("data_function must be a DataFunction")
+            self.listOfDataFunctions[df.name] = df
+        return data_function
 
     def timespan(self, time_span, timestep_size=None):
         if isinstance(time_span, TimeSpan):

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "vm.hpp"
 #include "mpn.hpp"
 
 using namespace ssa;
@@ -103,6 +104,10 @@ struct ssasdpd_engine {
     void *scan_tmp = nullptr;
     // chemistry constants
     Rxn *rx = nullptr;
+    // expression-VM programs of kind-4 reactions, data functions [n][n_df]
+    uint32_t *vm_code = nullptr;
+    double *vm_k = nullptr, *dfn = nullptr;
+    int n_code = 0, n_consts = 0, n_df = 0;
     int *jcN = nullptr, *irN = nullptr, *prN = nullptr, *jcG = nullptr, *irG = nullptr, *Ndense = nullptr;
     double *Dmat = nullptr;
     BcRule *bc = nullptr;
@@ -186,6 +191,7 @@ struct ssasdpd_engine {
         kd_work = nullptr;
         if (scan_tmp) (void)hipFree(scan_tmp);
         scan_tmp = nullptr;
+        dfree(vm_code); dfree(vm_k); dfree(dfn);
         dfree(rx); dfree(jcN); dfree(irN); dfree(prN); dfree(jcG); dfree(irG); dfree(Ndense); dfree(Dmat); dfree(bc);
         dfree(xx); dfree(rrate); dfree(srrate); dfree(sdrate); dfree(Ddiag); dfree(qtime); dfree(tmin); dfree(targ);
         dfree(qs); dfree(qwork); dfree(d_psum); dfree(d_words); dfree(d_expo); dfree(d_trace); dfree(derr);
@@ -298,7 +304,7 @@ struct ssasdpd_engine {
         dfree(tmin); dfree(targ);
         tmin = dalloc<double>(tree_nodes);
         targ = dalloc<int>(tree_nodes);
-        lds_bytes = ssa_lds_bytes(tree_nodes, S, R, T, nnzN_, nnzG_);
+        lds_bytes = ssa_lds_bytes(tree_nodes, S, R, T, nnzN_, nnzG_, n_code, n_consts);
         if (lds_bytes > 65536)
             throw HipError("event-loop constants exceed 64 KiB of LDS", hipErrorUnknown, __FILE__, __LINE__);
     }
@@ -372,6 +378,19 @@ struct ssasdpd_engine {
             }
             rx = dalloc<Rxn>(Rmax);
             h2d(st, rx, r.data(), Rmax);
+        }
+        if (d->n_code > 0) {
+            n_code = d->n_code;
+            n_consts = d->n_consts;
+            vm_code = dalloc<uint32_t>(n_code);
+            h2d(st, vm_code, d->code, n_code);
+            vm_k = dalloc<double>(std::max(n_consts, 1));
+            if (n_consts > 0) h2d(st, vm_k, d->consts, n_consts);
+        }
+        if (d->n_data_fn > 0) {
+            n_df = d->n_data_fn;
+            dfn = dalloc<double>((size_t)n * n_df);
+            h2d(st, dfn, d->data_fn, (size_t)n * n_df);
         }
         if (R > 0) {
             int nnz = d->jcN[R];
@@ -539,6 +558,7 @@ struct ssasdpd_engine {
         a.bvf = bvf; a.normal = normal; a.mass = mass; a.nu = nu; a.type = type; a.solid = solid; a.C = C; a.Q = Q;
         a.Dmat = Dmat; a.Ndense = Ndense; a.rx = rx; a.bc = bc; a.deg = deg; a.nidx = nidx; a.nr = nr; a.err = derr;
         a.fp32 = (flags & SSASDPD_FLAG_FP32_FORCES) ? 1 : 0;
+        a.vm_code = vm_code; a.vm_k = vm_k; a.data_fn = dfn; a.n_df = n_df;
         return a;
     }
 
@@ -562,6 +582,8 @@ struct ssasdpd_engine {
         a.endTime = static_domain ? nt * dt : a.t0 + dt;
         a.step = step;
         a.psum_out = d_psum;
+        a.vm_code = vm_code; a.vm_k = vm_k; a.vm_ncode = n_code; a.vm_nk = n_consts;
+        a.data_fn = dfn; a.n_df = n_df;
         return a;
     }
 
@@ -953,7 +975,7 @@ struct ssasdpd_ensemble {
     static size_t ssa_lds_bytes_for(const ssasdpd_engine *e, int p) {
         int n1 = (e->n + 64 * p - 1) / (64 * p), nodes = n1;
         while (n1 > 1) { n1 = (n1 + TREE_FAN - 1) / TREE_FAN; nodes += n1; }
-        return ssa_lds_bytes(nodes, e->S, e->R, e->T, e->nnzN_, e->nnzG_);
+        return ssa_lds_bytes(nodes, e->S, e->R, e->T, e->nnzN_, e->nnzG_, e->n_code, e->n_consts);
     }
     void set_pl(int p) {
         for (auto *e : eng) e->set_pl(p);
@@ -996,7 +1018,9 @@ struct ssasdpd_ensemble {
         if (args.empty()) return 0.f;
         h2d(st, d_args, args.data(), args.size());
         HIP_OK(hipEventRecord(e0, st));
-        rdme_take_step_multi(st, d_args, (int)args.size(), pl, max_sr, lds_max());
+        bool any_prog = false;
+        for (auto *e : eng) any_prog = any_prog || e->n_code > 0;
+        rdme_take_step_multi(st, d_args, (int)args.size(), pl, max_sr, lds_max(), any_prog);
         HIP_OK(hipEventRecord(e1, st));
         HIP_OK(hipStreamSynchronize(st));
         float ms = 0.f;
@@ -1057,8 +1081,25 @@ static int validate(const ssasdpd_desc *d) {
             return set_error(SSASDPD_ERR_BAD_DESC, "particle type out of range 1..n_types");
     int Rmax = std::max(d->n_reactions, d->n_chem_reactions);
     int Smax = std::max(d->n_species, d->n_chem_species);
+    if (d->n_code < 0 || d->n_consts < 0 || d->n_data_fn < 0 || (d->n_code > 0 && !d->code) ||
+        (d->n_consts > 0 && !d->consts) || (d->n_data_fn > 0 && !d->data_fn))
+        return set_error(SSASDPD_ERR_BAD_DESC, "bad program / data-function arrays");
+    if (d->n_code > (1 << 16) || d->n_consts > (1 << 16))
+        return set_error(SSASDPD_ERR_UNSUPPORTED, "at most 65536 program words and constants");
     for (int r = 0; r < Rmax; r++) {
         const ssasdpd_reaction &q = d->reactions[r];
+        if (q.kind == 4) {
+            // programs: the propensity (species = stochastic counts) and the
+            // deterministic rate (species = concentrations) must both check
+            if (d->n_reactions > 0 &&
+                (q.a < 0 || vm_check(d->code, d->n_code, d->n_consts, q.a, d->n_species, d->n_data_fn) <= 0))
+                return set_error(SSASDPD_ERR_BAD_DESC, "reaction " + std::to_string(r) + ": bad propensity program");
+            if (d->n_chem_reactions > 0 &&
+                (q.b < 0 || vm_check(d->code, d->n_code, d->n_consts, q.b, d->n_chem_species, d->n_data_fn) <= 0))
+                return set_error(SSASDPD_ERR_BAD_DESC,
+                                 "reaction " + std::to_string(r) + ": bad deterministic-rate program");
+            continue;
+        }
         if (q.kind < 0 || q.kind > 3 || q.a < 0 || q.a >= std::max(Smax, 1) || q.b < 0 || q.b >= std::max(Smax, 1))
             return set_error(SSASDPD_ERR_BAD_DESC, "bad reaction table entry");
     }
@@ -1322,6 +1363,50 @@ int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp,
     if (Frho) d2h(e->st, Frho, dR, n);
     HIP_OK(hipStreamSynchronize(e->st));
     dfree(dF); dfree(dB); dfree(dR); dfree(dQ);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_vm_eval(const uint32_t *code, int32_t n_code, const double *consts, int32_t n_consts, int32_t start,
+                    int32_t x_is_double, int32_t n_x, int32_t n_df, int32_t m, const double *x, const double *vol,
+                    const double *t, const double *df, double *out) {
+    GUARD_BEGIN
+    if (!code || n_code <= 0 || n_consts < 0 || (n_consts > 0 && !consts) || n_x < 0 || n_df < 0 || m < 0 ||
+        (m > 0 && (!vol || !t || !out || (n_x > 0 && !x) || (n_df > 0 && !df))))
+        return set_error(SSASDPD_ERR_BAD_DESC, "vm_eval: bad arguments");
+    if (start < 0 || vm_check(code, n_code, n_consts, start, n_x, n_df) <= 0)
+        return set_error(SSASDPD_ERR_BAD_DESC, "vm_eval: program fails the check");
+    if (m == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(SSASDPD_ERR_HIP, "no HIP device");
+    hipStream_t s;
+    HIP_OK(hipStreamCreate(&s));
+    const size_t nx = (size_t)m * n_x;
+    uint32_t *dc = dalloc<uint32_t>(n_code);
+    double *dk = dalloc<double>(std::max(n_consts, 1));
+    double *dxd = x_is_double ? dalloc<double>(std::max<size_t>(nx, 1)) : nullptr;
+    uint32_t *dxu = x_is_double ? nullptr : dalloc<uint32_t>(std::max<size_t>(nx, 1));
+    double *dv = dalloc<double>(m), *dt = dalloc<double>(m), *dout = dalloc<double>(m);
+    double *ddf = n_df > 0 ? dalloc<double>((size_t)m * n_df) : nullptr;
+    h2d(s, dc, code, n_code);
+    if (n_consts > 0) h2d(s, dk, consts, n_consts);
+    std::vector<uint32_t> xu;
+    if (x_is_double) {
+        if (nx) h2d(s, dxd, x, nx);
+    } else if (nx) {
+        xu.resize(nx);
+        for (size_t i = 0; i < nx; i++) xu[i] = (uint32_t)(int32_t)x[i];
+        h2d(s, dxu, xu.data(), nx);
+    }
+    h2d(s, dv, vol, m);
+    h2d(s, dt, t, m);
+    if (ddf) h2d(s, ddf, df, (size_t)m * n_df);
+    vm_eval_batch(s, dc, start, dk, x_is_double ? 1 : 0, n_x, n_df, m, dxd, dxu, dv, dt, ddf, dout);
+    HIP_OK(hipGetLastError());
+    d2h(s, out, dout, m);
+    HIP_OK(hipStreamSynchronize(s));
+    dfree(dc); dfree(dk); dfree(dxd); dfree(dxu); dfree(dv); dfree(dt); dfree(dout); dfree(ddf);
+    HIP_OK(hipStreamDestroy(s));
     return 0;
     GUARD_END
 }

@@ -147,6 +147,11 @@ struct SdpdArgs {
     const int *deg, *nidx;
     const double *nr;
     DevErr *err;
+    // expression-VM programs (kind-4 reactions) and data functions [n][n_df]
+    const uint32_t *vm_code;
+    const double *vm_k;
+    const double *data_fn;
+    int n_df;
 };
 void sdpd_predict(hipStream_t s, const SdpdArgs &a);
 void sdpd_forces(hipStream_t s, const SdpdArgs &a);
@@ -212,6 +217,13 @@ struct RdmeArgs {
     uint32_t *snap_xx;
     size_t bs_snap;
     uint32_t *snap_tot;
+    // expression-VM programs of kind-4 reactions (copied to LDS by the event
+    // loop) and data functions [n][n_df]
+    const uint32_t *vm_code;
+    const double *vm_k;
+    int vm_ncode, vm_nk;
+    const double *data_fn;
+    int n_df;
 };
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
@@ -219,8 +231,12 @@ void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void tree_rebuild(hipStream_t s, const RdmeArgs &a);
 // one launch over n_models argument blocks (device array), common PL and
 // MS/MR bucket (max over the models of max(S, R)), LDS = max over models
-void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds);
-size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG);
+void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds,
+                          bool any_prog);
+void vm_eval_batch(hipStream_t s, const uint32_t *code, int start, const double *K, int x_is_double, int n_x,
+                   int n_df, int m, const double *xd, const uint32_t *xu, const double *vol, const double *t,
+                   const double *df, double *out);
+size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG, int n_code = 0, int n_consts = 0);
 
 // ---- message-passing surrogate (mpn.hip) --------------------------------
 struct MpnGraph {

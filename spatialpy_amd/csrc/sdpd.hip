@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "engine.hpp"
+#include "vm.hpp"
 
 namespace ssa {
 
@@ -79,11 +80,19 @@ __global__ void k_predict(SdpdArgs a) {
     a.old_rho[i] = a.rho[i];
 }
 
+// generated det<name> functions (solver.py:160-190), called with
+// vol = mass/rho and t = current_step*dt (model.cpp:181-184); VM: kind-4
+// rows through the expression VM (only k_chem instantiates it)
+template <bool VM>
 __device__ __forceinline__ double chem_rhs(const SdpdArgs &a, int i, int r) {
-    // generated det<name> functions (solver.py:160-190)
     const Rxn &re = a.rx[r];
     if (re.mask && !((re.mask >> a.type[i]) & 1ULL)) return 0.0;
     const double *c = a.C + (size_t)i * a.Sc;
+    if constexpr (VM) {
+        if (re.kind == 4)
+            return vm_eval_f64(a.vm_code, re.b, a.vm_k, c, a.mass[i] / a.rho[i], (double)a.step * a.dt,
+                               a.data_fn ? a.data_fn + (size_t)i * a.n_df : nullptr);
+    }
     switch (re.kind) {
     case 0: return re.k;
     case 1: return re.k * c[re.a];
@@ -98,7 +107,10 @@ __device__ __forceinline__ double chem_rhs(const SdpdArgs &a, int i, int r) {
 // transport tensor, continuity) in FP32 from FP64-formed differences dx, dv
 // and sums them into FP64 accumulators.  The species flux Q stays FP64.  For
 // static domains only Q is observable, so the momentum part is skipped.
-template <typename Real>
+// CHEM: the reaction fluxes are added here; otherwise k_chem adds them
+// afterwards (models with custom deterministic rates), to the same Q in the
+// same order.
+template <typename Real, bool CHEM>
 __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
     if (chunk < 0) return;
@@ -193,12 +205,14 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
             }
         }
     }
-    for (int r = 0; r < a.Rc; r++) {
-        const double flux = chem_rhs(a, i, r);
-        for (int s = 0; s < Sc; s++) {
-            const long long kk = (long long)a.Rc * r + s;             // model.cpp:186
-            const int nk = kk < (long long)a.Sc * a.Rc ? a.Ndense[kk] : 0;
-            Q[s] += nk * flux;
+    if constexpr (CHEM) {
+        for (int r = 0; r < a.Rc; r++) {
+            const double flux = chem_rhs<false>(a, i, r);
+            for (int s = 0; s < Sc; s++) {
+                const long long kk = (long long)a.Rc * r + s;             // model.cpp:186
+                const int nk = kk < (long long)a.Sc * a.Rc ? a.Ndense[kk] : 0;
+                Q[s] += nk * flux;
+            }
         }
     }
     if (momentum) {
@@ -289,9 +303,35 @@ __global__ void k_bvf(SdpdArgs a) {
 static dim3 grid1(int n) { return xcd_grid((n + 255) / 256); }
 
 void sdpd_predict(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_predict, grid1(a.n), dim3(256), 0, s, a); }
+// The reaction fluxes of models with custom deterministic rates
+// (model.cpp:179-189), after k_forces has summed the pair fluxes into Q.
+__global__ void __launch_bounds__(256) k_chem(SdpdArgs a) {
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    const int i = chunk * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int Sc = a.Sc;
+    double *Q = a.Q + (size_t)i * Sc;
+    for (int r = 0; r < a.Rc; r++) {
+        const double flux = chem_rhs<true>(a, i, r);
+        for (int s = 0; s < Sc; s++) {
+            const long long kk = (long long)a.Rc * r + s;             // model.cpp:186
+            const int nk = kk < (long long)a.Sc * a.Rc ? a.Ndense[kk] : 0;
+            Q[s] += nk * flux;
+        }
+    }
+}
+
 void sdpd_forces(hipStream_t s, const SdpdArgs &a) {
-    if (a.fp32) hipLaunchKernelGGL(k_forces<float>, grid1(a.n), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_forces<double>, grid1(a.n), dim3(256), 0, s, a);
+    const bool vm = a.vm_code != nullptr && a.Rc > 0 && a.Sc > 0;
+    if (vm) {
+        if (a.fp32) hipLaunchKernelGGL((k_forces<float, false>), grid1(a.n), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_forces<double, false>), grid1(a.n), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_chem, grid1(a.n), dim3(256), 0, s, a);
+        return;
+    }
+    if (a.fp32) hipLaunchKernelGGL((k_forces<float, true>), grid1(a.n), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_forces<double, true>), grid1(a.n), dim3(256), 0, s, a);
 }
 void sdpd_correct(hipStream_t s, const SdpdArgs &a) {
     if (!a.static_domain) hipLaunchKernelGGL(k_correct, grid1(a.n), dim3(256), 0, s, a);

@@ -27,6 +27,7 @@
 #include <cmath>
 
 #include "engine.hpp"
+#include "vm.hpp"
 
 namespace ssa {
 
@@ -170,7 +171,9 @@ __device__ __forceinline__ double propensity(const Rxn &re, const uint32_t *xv, 
 }
 
 // ---------------------------------------------------------------------------
-// Rates (simulate_rdme.cpp:113-152), one thread per voxel.
+// Rates (simulate_rdme.cpp:113-152), one thread per voxel.  VM: kind-4
+// reactions through the expression VM.
+template <bool VM>
 __global__ void k_rates(RdmeArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
     if (chunk < 0) return;
@@ -183,9 +186,16 @@ __global__ void k_rates(RdmeArgs a) {
     uint32_t xv[MAXS];
     for (int s = 0; s < S; s++) xv[s] = xx[(size_t)i * S + s];
     double sr = 0.0;
+    const double *df = a.data_fn ? a.data_fn + (size_t)i * a.n_df : nullptr;
     for (int j = 0; j < R; j++) {
         double vol = a.mass[i] / a.rho[i];
-        double p = propensity(a.rx[j], xv, vol, a.type[i]);
+        const Rxn re = a.rx[j];
+        double p;
+        if (VM && re.kind == 4)   // simulate_rdme.cpp:124 evaluates at t = 0.0
+            p = (re.mask && !((re.mask >> a.type[i]) & 1ULL))
+                    ? 0.0 : vm_eval_u32(a.vm_code, re.a, a.vm_k, xx + (size_t)i * S, vol, 0.0, df);
+        else
+            p = propensity(re, xv, vol, a.type[i]);
         rrate[(size_t)i * R + j] = p;
         sr += p;
     }
@@ -206,7 +216,9 @@ __global__ void k_rates(RdmeArgs a) {
     sdrate[i] = sd;
 }
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj) {
-    hipLaunchKernelGGL(k_rates, dim3(xcd_grid((a.n + 255) / 256).x, n_traj), dim3(256), 0, s, a);
+    const dim3 g(xcd_grid((a.n + 255) / 256).x, n_traj);
+    if (a.vm_ncode > 0) hipLaunchKernelGGL(k_rates<true>, g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_rates<false>, g, dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,7 +496,7 @@ __device__ __forceinline__ double propensity_l(const Rxn &re, uint32_t x_l, doub
 // within a column); lane s has species s's stoichiometric change under each
 // reaction.
 template <int MR> struct LaneModel {
-    int kind, ra, rb;
+    int kind, ra, rb, pc;   // pc: program start of a kind-4 reaction
     double k;
     uint64_t mask;
     uint32_t dep;
@@ -504,6 +516,23 @@ __device__ __forceinline__ double propensity_par(const LaneModel<MR> &m, uint32_
     case 2: return ((m.k * xa) * xb) / vol;
     default: return ((m.k * xa) * (xa - 1)) / vol;
     }
+}
+// propensity_par with kind-4 rows (custom propensities) from the expression
+// VM, for the lanes in `want`: the counts are staged in LDS (vmx) so that
+// every lane's program can read any species; arguments as the generated
+// functions get them (counts, t, vol, data_fn, type).  Lanes outside `want`
+// hold values the callers mask off.
+template <int MR>
+__device__ __forceinline__ double propensity_vm(const LaneModel<MR> &m, uint32_t x_l, double vol, int tp, double tnow,
+                                                const double *df, uint64_t want, uint32_t *vmx,
+                                                const uint32_t *vcode, const double *vk, int lane) {
+    double p = propensity_par(m, x_l, vol, tp);
+    vmx[lane] = x_l;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (m.kind == 4 && ((want >> lane) & 1ULL))
+        p = (m.mask && !((m.mask >> tp) & 1ULL)) ? 0.0 : vm_eval_u32(vcode, m.pc, vk, vmx, vol, tnow, df);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    return p;
 }
 // Sum over the set bits of `bits` in ascending lane order of lane-held
 // values, added one by one (the reference's sequential order).
@@ -663,7 +692,9 @@ __device__ __noinline__ void snapshot_counts(const uint32_t *xx, uint32_t *snap_
 
 // The event loop of one trajectory on one wavefront.  b: its index in a
 // batch that shares one argument block (strides bs_*), else 0.
-template <int PL, int MS, int MR>
+// VM: kind-4 reactions (custom propensities) through the expression VM; the
+// kernels without it keep the mass-action table's register allocation.
+template <int PL, int MS, int MR, bool VM>
 __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -686,6 +717,8 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     Lds L;
     uint32_t *nbx;   // prefetched neighbour counts [64][MS]
     double *nbr;     // prefetched neighbour reaction rates [64][MR]
+    uint32_t *vcode = nullptr, *vmx = nullptr;   // VM programs, staged counts
+    double *vk = nullptr;                        // VM constants
     {
         size_t off = 0;
         L.lt = (double *)(smem + off); off = al16(off + sizeof(double) * a.tree_nodes);
@@ -700,6 +733,13 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         L.irG = (int *)(smem + off); off = al16(off + sizeof(int) * nnzG);
         nbx = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * WAVE * MS);
         nbr = (double *)(smem + off); off = al16(off + sizeof(double) * WAVE * MR);
+        if (VM && a.vm_ncode > 0) {
+            vcode = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * a.vm_ncode);
+            vk = (double *)(smem + off); off = al16(off + sizeof(double) * a.vm_nk);
+            vmx = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * WAVE);
+            for (int i = lane; i < a.vm_ncode; i += WAVE) vcode[i] = a.vm_code[i];
+            for (int i = lane; i < a.vm_nk; i += WAVE) vk[i] = a.vm_k[i];
+        }
         for (int i = lane; i < a.tree_nodes; i += WAVE) { L.lt[i] = tmin_[i]; L.lv[i] = targ_[i]; }
         for (int i = lane; i < R; i += WAVE) L.rx[i] = a.rx[i];
         for (int i = lane; i < S * T; i += WAVE) L.Dmat[i] = a.Dmat[i];
@@ -721,10 +761,11 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         if (lane == k) { tsh.lo_l = a.lev_off[k]; tsh.ln_l = a.lev_n[k]; }
     LaneModel<MR> lm;
     {
-        lm.kind = 0; lm.ra = 0; lm.rb = 0; lm.k = 0.0; lm.mask = 0; lm.dep = 0;
+        lm.kind = 0; lm.ra = 0; lm.rb = 0; lm.pc = 0; lm.k = 0.0; lm.mask = 0; lm.dep = 0;
         if (lane < R) {
             const Rxn r = L.rx[lane];
             lm.kind = r.kind; lm.ra = r.a; lm.rb = r.b; lm.k = r.k; lm.mask = r.mask;
+            if (r.kind == 4) { lm.pc = r.a; lm.ra = 0; lm.rb = 0; }
         }
         if (lane < S + R)
             for (int k = L.jcG[lane]; k < L.jcG[lane + 1]; k++) lm.dep |= 1u << L.irG[k];
@@ -749,6 +790,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     rng.exhausted = false;
     rng.init(lane);
     const bool textbook = a.textbook != 0;
+    const bool has_prog = VM && a.vm_ncode > 0;
     const bool lead = lane == 0;
     const bool s_lane = lane < S, r_lane = lane < R;
 
@@ -813,6 +855,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         Sib<PL> sib;
         sib.load(qtime_, n, sv / (WAVE * PL), lane);
         const double vol = a.vol[sv];
+        const double *const df_sv = a.data_fn ? a.data_fn + (size_t)sv * a.n_df : nullptr;
         double sr = srrate_[sv], sd = sdrate_[sv];
         const int tsv = a.type[sv];
         const int dg_l = a.deg[sv];
@@ -848,10 +891,17 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (re >= R) {
                 if (cum != sr) {
                     double rd = 0.0;
-                    for (int j = 0; j < R; j++) {
-                        double p = propensity_l(L.rx[j], xs, vol, tsv);
-                        if (lane == j) rr = p;
-                        rd += p;
+                    if (has_prog) {
+                        const uint64_t all_r = (1ULL << R) - 1ULL;
+                        const double p = propensity_vm(lm, xs, vol, tsv, tt, df_sv, all_r, vmx, vcode, vk, lane);
+                        rd = ordered_sum(0.0, p, all_r);
+                        if (r_lane) rr = p;
+                    } else {
+                        for (int j = 0; j < R; j++) {
+                            double p = propensity_l(L.rx[j], xs, vol, tsv);
+                            if (lane == j) rr = p;
+                            rd += p;
+                        }
                     }
                     sr = rd;
                     if (r_lane) rrate_[(size_t)sv * R + lane] = rr;
@@ -882,7 +932,8 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             // dependency-graph column S+re: all propensities at once, the
             // changes added in the column's order
             const uint64_t dep = (uint32_t)__builtin_amdgcn_readlane((int)lm.dep, S + re);
-            const double p_new = propensity_par(lm, xs, vol, tsv);
+            const double p_new = has_prog ? propensity_vm(lm, xs, vol, tsv, tt, df_sv, dep, vmx, vcode, vk, lane)
+                                          : propensity_par(lm, xs, vol, tsv);
             const double rd = ordered_sum(0.0, p_new - rr, dep);
             if ((dep >> lane) & 1ULL) rr = p_new;
             sr += rd;
@@ -1028,8 +1079,15 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (self) xs = xd;
             if (R > 0 && !self) {
                 const uint64_t dep = (uint32_t)__builtin_amdgcn_readlane((int)lm.dep, spec);
-                const double p_s = propensity_par(lm, xs, vol, tsv);
-                const double p_d = propensity_par(lm, xd, vol, tds);
+                double p_s, p_d;
+                if (has_prog) {   // destination: its counts, data_fn and type, the source's vol
+                    const double *df_d = a.data_fn ? a.data_fn + (size_t)dest * a.n_df : nullptr;
+                    p_s = propensity_vm(lm, xs, vol, tsv, tt, df_sv, dep, vmx, vcode, vk, lane);
+                    p_d = propensity_vm(lm, xd, vol, tds, tt, df_d, dep, vmx, vcode, vk, lane);
+                } else {
+                    p_s = propensity_par(lm, xs, vol, tsv);
+                    p_d = propensity_par(lm, xd, vol, tds);
+                }
                 const double rd = ordered_sum(0.0, p_s - rr, dep);
                 const double rrdel = ordered_sum(0.0, p_d - rrd, dep);
                 if ((dep >> lane) & 1ULL) { rr = p_s; rrd = p_d; }
@@ -1037,15 +1095,22 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                 srd += rrdel;
             } else if (R > 0) {
                 double rd = 0.0, rrdel = 0.0;
+                // self: source and destination are one voxel, so one set of
+                // propensities serves both evaluations
+                double pv = 0.0;
+                if (has_prog) {
+                    const uint64_t dep = (uint32_t)__builtin_amdgcn_readlane((int)lm.dep, spec);
+                    pv = propensity_vm(lm, xs, vol, tsv, tt, df_sv, dep, vmx, vcode, vk, lane);
+                }
                 for (int k = L.jcG[spec]; k < L.jcG[spec + 1]; k++) {
                     const int j = L.irG[k];
                     double old = rl_d(rr, j);
-                    const double p = propensity_l(L.rx[j], xs, vol, tsv);
+                    const double p = has_prog ? rl_d(pv, j) : propensity_l(L.rx[j], xs, vol, tsv);
                     if (lane == j) rr = p;
                     rd += p - old;
                     if (self) rrd = rr;
                     old = rl_d(rrd, j);
-                    const double pd = propensity_l(L.rx[j], xd, vol, tds);
+                    const double pd = has_prog ? rl_d(pv, j) : propensity_l(L.rx[j], xd, vol, tds);
                     if (lane == j) rrd = pd;
                     rrdel += pd - old;
                     if (self) rr = rrd;
@@ -1119,58 +1184,81 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     if (lead) *qs_ = q;
 }
 
-template <int PL, int MS, int MR>
+template <int PL, int MS, int MR, bool VM>
 __global__ void __launch_bounds__(WAVE) k_ssa(RdmeArgs a) {
-    ssa_run<PL, MS, MR>(a, blockIdx.x);
+    ssa_run<PL, MS, MR, VM>(a, blockIdx.x);
 }
 
 // Ensembles of different models: workgroup g runs the trajectory described
 // by args[g] (its own domain, chemistry, queue and stream).
-template <int PL, int MS, int MR>
+template <int PL, int MS, int MR, bool VM>
 __global__ void __launch_bounds__(WAVE) k_ssa_multi(const RdmeArgs *__restrict__ args) {
-    ssa_run<PL, MS, MR>(args[blockIdx.x], 0);
+    ssa_run<PL, MS, MR, VM>(args[blockIdx.x], 0);
 }
 
-template <int MS, int MR>
+template <int MS, int MR, bool VM>
 static void launch_multi_ms(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, size_t lds) {
     const dim3 g(n_models);
     switch (pl) {
-    case 1: hipLaunchKernelGGL((k_ssa_multi<1, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
-    case 2: hipLaunchKernelGGL((k_ssa_multi<2, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
-    case 4: hipLaunchKernelGGL((k_ssa_multi<4, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
-    case 8: hipLaunchKernelGGL((k_ssa_multi<8, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
-    case 16: hipLaunchKernelGGL((k_ssa_multi<16, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
-    default: hipLaunchKernelGGL((k_ssa_multi<32, MS, MR>), g, dim3(WAVE), lds, s, d_args); break;
+    case 1: hipLaunchKernelGGL((k_ssa_multi<1, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
+    case 2: hipLaunchKernelGGL((k_ssa_multi<2, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
+    case 4: hipLaunchKernelGGL((k_ssa_multi<4, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
+    case 8: hipLaunchKernelGGL((k_ssa_multi<8, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
+    case 16: hipLaunchKernelGGL((k_ssa_multi<16, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
+    default: hipLaunchKernelGGL((k_ssa_multi<32, MS, MR, VM>), g, dim3(WAVE), lds, s, d_args); break;
     }
 }
 
-void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds) {
-    if (max_sr <= 4) launch_multi_ms<4, 4>(s, d_args, n_models, pl, lds);
-    else if (max_sr <= 8) launch_multi_ms<8, 8>(s, d_args, n_models, pl, lds);
-    else launch_multi_ms<16, 16>(s, d_args, n_models, pl, lds);
+// Models with custom propensities use the widest per-lane tables (S, R <= 16)
+// and the kernels with the expression VM.
+void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, int pl, int max_sr, size_t lds,
+                          bool any_prog) {
+    if (any_prog) launch_multi_ms<16, 16, true>(s, d_args, n_models, pl, lds);
+    else if (max_sr <= 4) launch_multi_ms<4, 4, false>(s, d_args, n_models, pl, lds);
+    else if (max_sr <= 8) launch_multi_ms<8, 8, false>(s, d_args, n_models, pl, lds);
+    else launch_multi_ms<16, 16, false>(s, d_args, n_models, pl, lds);
 }
 
-template <int MS, int MR> static void launch_ms(hipStream_t s, const RdmeArgs &a, int nt) {
+template <int MS, int MR, bool VM> static void launch_ms(hipStream_t s, const RdmeArgs &a, int nt) {
     const dim3 g(nt);
     switch (a.pl) {
-    case 1: hipLaunchKernelGGL((k_ssa<1, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
-    case 2: hipLaunchKernelGGL((k_ssa<2, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
-    case 4: hipLaunchKernelGGL((k_ssa<4, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
-    case 8: hipLaunchKernelGGL((k_ssa<8, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
-    case 16: hipLaunchKernelGGL((k_ssa<16, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
-    default: hipLaunchKernelGGL((k_ssa<32, MS, MR>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 1: hipLaunchKernelGGL((k_ssa<1, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 2: hipLaunchKernelGGL((k_ssa<2, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 4: hipLaunchKernelGGL((k_ssa<4, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 8: hipLaunchKernelGGL((k_ssa<8, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    case 16: hipLaunchKernelGGL((k_ssa<16, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
+    default: hipLaunchKernelGGL((k_ssa<32, MS, MR, VM>), g, dim3(WAVE), a.lds_bytes, s, a); break;
     }
 }
 
 void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj) {
     const int m = a.S > a.R ? a.S : a.R;
-    if (m <= 4) launch_ms<4, 4>(s, a, n_traj);
-    else if (m <= 8) launch_ms<8, 8>(s, a, n_traj);
-    else launch_ms<16, 16>(s, a, n_traj);
+    if (a.vm_ncode > 0) launch_ms<16, 16, true>(s, a, n_traj);
+    else if (m <= 4) launch_ms<4, 4, false>(s, a, n_traj);
+    else if (m <= 8) launch_ms<8, 8, false>(s, a, n_traj);
+    else launch_ms<16, 16, false>(s, a, n_traj);
+}
+
+// The expression VM over m argument sets (ssasdpd_vm_eval).
+__global__ void k_vm_eval(const uint32_t *code, int start, const double *K, int x_is_double, int n_x, int n_df,
+                          int m, const double *xd, const uint32_t *xu, const double *vol, const double *t,
+                          const double *df, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double *dfi = n_df > 0 ? df + (size_t)i * n_df : nullptr;
+    out[i] = x_is_double ? vm_eval_f64(code, start, K, xd + (size_t)i * n_x, vol[i], t[i], dfi)
+                         : vm_eval_u32(code, start, K, xu + (size_t)i * n_x, vol[i], t[i], dfi);
+}
+void vm_eval_batch(hipStream_t s, const uint32_t *code, int start, const double *K, int x_is_double, int n_x,
+                   int n_df, int m, const double *xd, const uint32_t *xu, const double *vol, const double *t,
+                   const double *df, double *out) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_vm_eval, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, code, start, K, x_is_double,
+                       n_x, n_df, m, xd, xu, vol, t, df, out);
 }
 
 // LDS bytes of the event-loop kernel for a given model.
-size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG) {
+size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG, int n_code, int n_consts) {
     auto a16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
     size_t off = 0;
     off = a16(off + sizeof(double) * tree_nodes);
@@ -1184,6 +1272,11 @@ size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG) {
     off = a16(off + sizeof(int) * nnzG);
     off = a16(off + sizeof(uint32_t) * 64 * 16);
     off = a16(off + sizeof(double) * 64 * 16);
+    if (n_code > 0) {
+        off = a16(off + sizeof(uint32_t) * n_code);
+        off = a16(off + sizeof(double) * n_consts);
+        off = a16(off + sizeof(uint32_t) * 64);
+    }
     return off;
 }
 

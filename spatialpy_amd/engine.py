@@ -220,3 +220,27 @@ def fr_search(pts, q, dim, h):
     _abi.check(lib, lib.ssasdpd_fr_search(_p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), m, dim, h,
                                           _p(rp, C.c_int64), _p(idx, C.c_int32), _p(r2, C.c_double)), "fr_search")
     return rp, idx, r2
+
+
+def vm_eval(code, consts, start, x, vol, t, df=None, x_is_double=False):
+    """Evaluate one compiled propensity program (spatialpy_amd.expression)
+    on the GPU with the engine's interpreter for m argument sets: x [m][n_x]
+    (integral values when x_is_double is False), vol and t [m], df [m][n_df]."""
+    lib = _abi.load_library()
+    code = np.ascontiguousarray(code, dtype=np.uint32)
+    consts = np.ascontiguousarray(consts, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    m = x.shape[0]
+    x = x.reshape(m, -1)
+    vol = np.ascontiguousarray(np.broadcast_to(np.asarray(vol, dtype=np.float64), (m,)))
+    t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (m,)))
+    n_df = 0
+    if df is not None:
+        df = np.ascontiguousarray(df, dtype=np.float64).reshape(m, -1)
+        n_df = df.shape[1]
+    out = np.zeros(m)
+    _abi.check(lib, lib.ssasdpd_vm_eval(_p(code, C.c_uint32), code.size, _p(consts, C.c_double) if consts.size else None,
+                                        consts.size, int(start), int(bool(x_is_double)), x.shape[1], n_df, m,
+                                        _p(x, C.c_double), _p(vol, C.c_double), _p(t, C.c_double),
+                                        _p(df, C.c_double) if n_df else None, _p(out, C.c_double)), "vm_eval")
+    return out

@@ -12,8 +12,8 @@ import numpy as np
 
 from . import _abi
 
-# propensity kinds (reaction.py:260-313)
-K_ZERO, K_FIRST, K_AB, K_2X = 0, 1, 2, 3
+# propensity kinds (reaction.py:260-313); 4: expression-VM program
+K_ZERO, K_FIRST, K_AB, K_2X, K_PROGRAM = 0, 1, 2, 3, 4
 
 
 @dataclass
@@ -41,7 +41,7 @@ class SystemArrays:
     # stochastic chemistry
     u0: np.ndarray = None         # [n,S] uint32
     diffusion_matrix: np.ndarray = None   # [S,T] float64 (species-major)
-    reactions: list = field(default_factory=list)   # (kind, a, b, k, type_mask)
+    reactions: list = field(default_factory=list)   # (kind, a, b, k, type_mask[, reactants, products])
     jcN: np.ndarray = None
     irN: np.ndarray = None
     prN: np.ndarray = None
@@ -53,6 +53,17 @@ class SystemArrays:
     C0: np.ndarray = None         # [n,Sc] float64 or None (= u0)
     bcs: list = field(default_factory=list)   # dicts: lo, hi, type, target, species, value
     flags: int = 0
+    # custom propensities (kind-4 rows): VM code and constants; any custom
+    # reaction makes the dependency graph dense (core/model.py:187-196)
+    custom_propensities: bool = False
+    vm_code: np.ndarray = None    # [n_code] uint32
+    vm_consts: np.ndarray = None  # [n_consts] float64
+    data_fn: np.ndarray = None    # [n, n_data_fn] float64
+    # the C++ text of each reaction's functions and the parameter values
+    # (kept for writing the reference harness's model)
+    prop_cpp: list = None
+    ode_cpp: list = None
+    param_values: list = None
 
     @property
     def n(self):
@@ -89,11 +100,17 @@ class SystemArrays:
                 setattr(self, k, i4(getattr(self, k)))
         if self.C0 is not None:
             self.C0 = f8(self.C0)
+        if self.vm_code is not None:
+            self.vm_code = np.ascontiguousarray(self.vm_code, dtype=np.uint32)
+            self.vm_consts = f8(self.vm_consts if self.vm_consts is not None else np.zeros(0))
+        if self.data_fn is not None:
+            self.data_fn = f8(self.data_fn).reshape(self.n, -1)
         return self
 
     def build_sparse(self):
         """Stoichiometry CSC and dependency graph CSC exactly as
-        core/model.py:187-268 builds them for mass-action models."""
+        core/model.py:187-268 builds them (all ones when any reaction has a
+        custom propensity)."""
         S, R = self.n_species, self.n_reactions
         reactants = [rx[5] if len(rx) > 5 else _default_reactants(rx) for rx in self.reactions]
         products = [rx[6] if len(rx) > 6 else {} for rx in self.reactions]
@@ -114,11 +131,13 @@ class SystemArrays:
         self.N_dense = N.astype(np.int32)
         # dependency graph (model.py:187-250)
         raw = np.zeros((R, R + S))
+        if self.custom_propensities:
+            raw[:] = 1
         spec_to_rx = [[r for r in range(R) if s in reactants[r]] for s in range(S)]
         for s in range(S):
             for r in spec_to_rx[s]:
                 raw[r, s] = 1
-        for i in range(R):
+        for i in range(R if not self.custom_propensities else 0):
             deps = set()
             for s in list(reactants[i].keys()) + list(products[i].keys()):
                 deps.update(spec_to_rx[s])
@@ -210,6 +229,13 @@ class DescHandle:
         d.bcs = C.cast(bcs, C.POINTER(_abi.BC))
         d.n_output_steps = int(sa.output_steps.size)
         d.output_steps = _abi.ptr(sa.output_steps, C.c_uint32)
+        if sa.vm_code is not None and sa.vm_code.size:
+            d.n_code, d.n_consts = int(sa.vm_code.size), int(sa.vm_consts.size)
+            d.code = _abi.ptr(sa.vm_code, C.c_uint32)
+            d.consts = _abi.ptr(sa.vm_consts, C.c_double) if sa.vm_consts.size else None
+        if sa.data_fn is not None and sa.data_fn.shape[1] > 0:
+            d.n_data_fn = int(sa.data_fn.shape[1])
+            d.data_fn = _abi.ptr(sa.data_fn, C.c_double)
         self.desc = d
 
     def byref(self):
@@ -289,3 +315,31 @@ def write_harness_dir(sa: SystemArrays, path, seed, run_steps=None, debug_flag=0
     w("bc.f64", bd, "<f8")
     w("bc.i32", bi, "<i4")
     w("out_steps.u32", sa.output_steps, "<u4")
+    if sa.data_fn is not None and sa.data_fn.shape[1] > 0:
+        # the generated input_data_fn layout: [function][particle]
+        w("data_fn.f64", sa.data_fn.T, "<f8")
+        with open(os.path.join(path, "meta.txt"), "a") as f:
+            f.write(f"n_data_fn {sa.data_fn.shape[1]}\n")
+
+
+def harness_propensity_source(sa: SystemArrays):
+    """C++ source of a model's reaction functions in the form the
+    reference generator writes them (solver.py:160-190 det<name>, 300-310
+    parameter definitions, 344-373 propensities), one extern "C" symbol per
+    reaction (prop_<r>, det_<r>), for oracle/ref_harness.cpp."""
+    out = ["#include <math.h>", "#include <stddef.h>"]
+    for i, v in enumerate(sa.param_values or []):
+        out.append(f"static const double P{i} = {float(v)!r};")
+
+    def body(expr, mask):
+        if not mask:
+            return f"return {expr};"
+        conds = "||".join(f"sd == {t}" for t in range(64) if (mask >> t) & 1)
+        return f"if({conds}){{\nreturn {expr};\n}}else{{\n\treturn 0.0;}}"
+    for r, rx in enumerate(sa.reactions):
+        mask = int(rx[4])
+        out.append(f'extern "C" double prop_{r}(const int *x, double t, const double vol, const double *data_fn, '
+                   f'int sd)\n{{\n{body(sa.prop_cpp[r], mask)}\n}}')
+        out.append(f'extern "C" double det_{r}(const double *x, double t, const double vol, const double *data_fn, '
+                   f'int sd)\n{{\n{body(sa.ode_cpp[r], mask)}\n}}')
+    return "\n".join(out) + "\n"

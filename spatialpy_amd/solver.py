@@ -8,10 +8,10 @@ import time
 
 import numpy as np
 
-from . import _abi
+from . import _abi, expression
 from .core.errors import ModelError, SimulationError
 from .core.model import Parameter, Species
-from .lowering import SystemArrays, K_ZERO, K_FIRST, K_AB, K_2X, output_schedule
+from .lowering import SystemArrays, output_schedule
 
 
 class Result:
@@ -119,6 +119,15 @@ class Solver:
 
     # -- lowering (solver.py:100-419, core/model.py:187-268)
     def compile(self, debug=False, profile=False):
+        """Lower the model and create the GPU engine (the reference's
+        build step, solver.py:443-507)."""
+        self.sa = self.lower()
+        from .engine import Engine
+        self.engine = Engine(self.sa)
+        self.is_compiled = True
+
+    def lower(self):
+        """The model as engine arrays (no GPU needed)."""
         m = self.model
         params = m.compile_prep()
         d = m.domain
@@ -145,28 +154,35 @@ class Solver:
                     continue
                 if sp.restrict_to is None or name in sp.restrict_to:
                     D[i, j - 1] = float(coef)
-        rx = []
+        # reactions: the propensity and deterministic-rate strings become C++
+        # text (solver.py:160-190, 344-373 via BuildExpression.getexpr_cpp),
+        # read with C types; the four mass-action shapes run from the table,
+        # anything else as VM programs
+        dfs = list(m.listOfDataFunctions.values())
+        ns = expression.model_namespace(list(m.listOfSpecies), list(m.listOfParameters), [f.name for f in dfs])
+        cparams = {f"P{i}": float(params[name]) for i, name in enumerate(m.listOfParameters)}
+        prog = expression.Program()
+        rx, prop_cpp, ode_cpp = [], [], []
         for r in m.listOfReactions.values():
-            if not r.massaction:
-                raise SimulationError(f"Reaction {r.name}: custom propensity functions are not supported "
-                                      "by the GPU engine yet (mass-action reactions only)")
-            k = r.marate
-            if isinstance(k, Parameter):
-                k = k.value if k.value is not None else params[k.name]
-            elif isinstance(k, str):
-                k = params[k] if k in params else float(eval(k, {"__builtins__": {}, "math": math}, dict(params)))
-            k = float(k)
-            reac = {sindex(s): c for s, c in r.reactants.items()}
-            prod = {sindex(s): c for s, c in r.products.items()}
-            keys = list(reac.keys())
-            if not reac:
-                kind, a, b = K_ZERO, 0, 0
-            elif len(reac) == 1 and reac[keys[0]] == 2:
-                kind, a, b = K_2X, keys[0], 0
-            elif len(reac) == 1:
-                kind, a, b = K_FIRST, keys[0], 0
+            try:
+                pc = expression.cpp_text(r.propensity_function, ns)
+                oc = expression.cpp_text(r.ode_propensity_function, ns)
+                tree_p = expression.parse_c(pc, cparams, S, len(dfs), expression.I)
+                tree_o = expression.parse_c(oc, cparams, S, len(dfs), expression.D)
+            except (SyntaxError, expression.ExpressionError) as e:
+                raise SimulationError(f"Reaction {r.name}: {e}") from e
+            prop_cpp.append(pc)
+            ode_cpp.append(oc)
+            shape = expression.table_shape(tree_p, tree_o)
+            if shape is not None:
+                kind, a, b, k = shape
             else:
-                kind, a, b = K_AB, keys[0], keys[1]
+                try:
+                    kind, a, b, k = expression.K_PROGRAM, prog.add(tree_p), prog.add(tree_o), 0.0
+                except expression.ExpressionError as e:
+                    raise SimulationError(f"Reaction {r.name}: {e}") from e
+            reac = {sindex(sp): c for sp, c in r.reactants.items()}
+            prod = {sindex(sp): c for sp, c in r.products.items()}
             mask = 0
             if r.restrict_to:
                 for t in r.restrict_to:
@@ -211,13 +227,19 @@ class Solver:
             bbox=(d.xlim[0], d.xlim[1], d.ylim[0], d.ylim[1], d.zlim[0], d.zlim[1]),
             u0=np.ascontiguousarray(m.u0.T).astype(np.uint32) if S else None,
             diffusion_matrix=D if S else None, reactions=rx, enable_rdme=m.enable_rdme, enable_pde=m.enable_pde,
-            bcs=bcs, flags=self.flags)
+            bcs=bcs, flags=self.flags,
+            custom_propensities=any(not r.massaction for r in m.listOfReactions.values()))
+        if prog.code:
+            sa.vm_code, sa.vm_consts = prog.arrays()
+        if dfs and S:
+            # solver.py:192-198, 239-249: each function's value at the initial
+            # positions (one column per function)
+            sa.data_fn = np.array([[float(f.map([p[0], p[1], p[2]])) for f in dfs] for p in x],
+                                  dtype=np.float64).reshape(len(x), len(dfs))
+        sa.prop_cpp, sa.ode_cpp, sa.param_values = prop_cpp, ode_cpp, list(cparams.values())
         if S:
             sa.build_sparse()
-        self.sa = sa.normalized()
-        from .engine import Engine
-        self.engine = Engine(self.sa)
-        self.is_compiled = True
+        return sa.normalized()
 
     def run(self, number_of_trajectories=1, seed=None, timeout=None, number_of_threads=None, debug=False,
             profile=False, verbose=True):

@@ -279,3 +279,43 @@ def gene_network_3d(n_target=8000, seed=0, t_end=200, dt=1.0):
                       diffusion_matrix=Ds, reactions=rxns)
     sa.build_sparse()
     return sa.normalized()
+
+
+def custom_chemistry_2d(numx=12, numy=12, t_end=2, num_points=5, dt=0.1, seed=0, libm=False):
+    """A static 2D model built through the Model API whose reactions are
+    custom propensity functions (Reaction(propensity_function=...)): integer
+    arithmetic of the generated C++ (A*(A-1)/2, B/3), a data function, type
+    restrictions, sqrt, and with libm=True time dependence and pow/exp.  Two
+    particle types (x > 0.5 is type 2)."""
+    from .core import model as M
+    from .core.domain import Domain
+    from .solver import Solver
+
+    class Gradient(M.DataFunction):
+        def map(self, point):
+            return 1.0 + 2.0 * point[0]
+
+    m = M.Model("custom_chemistry")
+    d = Domain.create_2D_domain(xlim=(0, 1), ylim=(0, 1), numx=numx, numy=numy, type_id=1, fixed=True)
+    d.type_id[d.vertices[:, 0] > 0.5] = "type_2"
+    m.add_domain(d)
+    m.add_species([M.Species("A", 0.05), M.Species("B", 0.02), M.Species("C", 0.01, restrict_to=[1])])
+    m.add_parameter([M.Parameter("k1", expression=40.0), M.Parameter("k2", expression=0.5),
+                     M.Parameter("K", expression=6.0), M.Parameter("kd", expression=0.05)])
+    m.add_data_function(Gradient(name="grad"))
+    birth = "k1*vol*grad*exp(-t/4)" if libm else "k1*vol*grad"
+    conv = "k2*A*pow(A,2)/(K*K+pow(A,2))" if libm else "k2*A/(1+A/K)"
+    m.add_reaction([
+        M.Reaction("birth", reactants={}, products={"A": 1}, propensity_function=birth),
+        M.Reaction("conv", reactants={"A": 1}, products={"B": 1}, propensity_function=conv),
+        M.Reaction("dimer", reactants={"A": 2}, products={"C": 1}, propensity_function="A*(A-1)/2*kd",
+                   restrict_to=[1]),
+        M.Reaction("decay", reactants={"B": 1}, products={}, propensity_function="kd*B + B/3"),
+        M.Reaction("back", reactants={"C": 1}, products={"A": 1}, rate="kd"),
+        M.Reaction("recycle", reactants={"B": 1}, products={"A": 1}, propensity_function="sqrt(B)*k2",
+                   restrict_to=[2]),
+    ])
+    m.add_initial_condition(M.ScatterInitialCondition("A", 300, seed=seed))
+    m.add_initial_condition(M.ScatterInitialCondition("B", 100, seed=seed + 1))
+    m.timespan(M.TimeSpan.linspace(t=t_end, num_points=num_points, timestep_size=dt))
+    return Solver(m).lower()

@@ -38,13 +38,33 @@ CASES = {
     "c5_cylinder_small": dict(gen="cylinder_3d",
                               params=dict(n_target=1200, nt=10, out_every=5, seed=4),
                               seed=9, kw=dict(force_step=6, nbr_step=6)),
+    # custom propensity functions (the reference compiles their generated
+    # C++; the harness loads it from props.so)
+    "c6_custom_chemistry": dict(gen="custom_chemistry_2d", params=dict(seed=3), seed=13, kw={}),
+    "c6_custom_chemistry_libm": dict(gen="custom_chemistry_2d", params=dict(seed=3, libm=True), seed=13, kw={}),
 }
+
+
+def add_propensity_library(sa, d):
+    """Compile the model's generated reaction functions the way the
+    reference builds its model file (g++ -std=c++14, no optimisation) into
+    d/props.so and point the harness at it."""
+    if sa.vm_code is None or sa.prop_cpp is None:
+        return
+    src = os.path.join(d, "props.cpp")
+    with open(src, "w") as f:
+        f.write(lowering.harness_propensity_source(sa))
+    subprocess.run(["g++", "-std=c++14", "-shared", "-fPIC", "-w", src, "-o", os.path.join(d, "props.so")],
+                   check=True)
+    with open(os.path.join(d, "meta.txt"), "a") as f:
+        f.write("prop_lib props.so\n")
 
 
 def make_case(name, spec):
     sa = getattr(workloads, spec["gen"])(**spec["params"])
     with tempfile.TemporaryDirectory() as d:
         lowering.write_harness_dir(sa, d, seed=spec["seed"], **spec["kw"])
+        add_propensity_library(sa, d)
         r = subprocess.run([HARNESS, d], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: harness failed: {r.stdout[-500:]} {r.stderr[-500:]}")

@@ -69,11 +69,18 @@ def test_propensity_expressions_match_reference_strings():
     assert sp.Reaction("r", reactants={}, products={A: 1}, rate=k1).propensity_expression() == "(k1*vol)"
 
 
-def test_custom_propensity_is_rejected_with_clear_error():
+def test_custom_propensities_lower_to_table_rows_or_programs():
     m = birth_death_model()
     m.add_reaction(sp.Reaction("custom", reactants={"Rabbits": 1}, products={}, propensity_function="k_death*Rabbits"))
-    with pytest.raises(sp.SimulationError, match="custom propensity"):
-        _NoEngineSolver(m).compile()
+    m.add_reaction(sp.Reaction("halve", reactants={"Rabbits": 1}, products={}, propensity_function="Rabbits/2"))
+    sa = sp.Solver(m).lower()
+    # the same C text as a mass-action first-order row runs from the table;
+    # int division needs a program
+    assert [r[0] for r in sa.reactions] == [0, 1, 1, 4]
+    assert sa.custom_propensities and sa.irG.size == sa.n_reactions * (sa.n_reactions + sa.n_species)
+    m.add_reaction(sp.Reaction("bad", reactants={"Rabbits": 1}, products={}, propensity_function="k_death*q"))
+    with pytest.raises(sp.SimulationError, match="invalid names"):
+        sp.Solver(m).lower()
 
 
 @pytest.mark.gpu
