@@ -454,18 +454,21 @@ template <int CTRL, int RMASK>
 __device__ __forceinline__ double dpp_min(double t) {
     return fmin(t, dpp_d<CTRL, RMASK>(t));
 }
-// argmin over the wavefront; every lane gets the result.  The minimum time
-// is reduced alone (DPP + v_min_f64); the winner is the lowest lane holding
-// it, and lanes hold increasing voxel ranges, so ties go to the lower voxel.
-__device__ __forceinline__ void wave_argmin_dpp(double &t, int &v) {
-    double m = t;
+// Minimum over the wavefront (DPP + v_min_f64), returned to every lane.
+__device__ __forceinline__ double wave_min_dpp(double m) {
     m = dpp_min<0xB1, 0xF>(m);    // quad_perm [1,0,3,2]
     m = dpp_min<0x4E, 0xF>(m);    // quad_perm [2,3,0,1]
     m = dpp_min<0x141, 0xF>(m);   // row_half_mirror
     m = dpp_min<0x140, 0xF>(m);   // row_mirror
     m = dpp_min<0x142, 0xA>(m);   // row_bcast15 into rows 1, 3
     m = dpp_min<0x143, 0xC>(m);   // row_bcast31 into rows 2, 3
-    m = rl_d(m, 63);
+    return rl_d(m, 63);
+}
+// argmin over the wavefront; every lane gets the result.  The minimum time
+// is reduced alone (DPP + v_min_f64); the winner is the lowest lane holding
+// it, and lanes hold increasing voxel ranges, so ties go to the lower voxel.
+__device__ __forceinline__ void wave_argmin_dpp(double &t, int &v) {
+    const double m = wave_min_dpp(t);
     const unsigned long long hit = __ballot(t == m);
     const int w = hit ? (int)__builtin_ctzll(hit) : 0;
     v = __builtin_amdgcn_readlane(v, w);
@@ -570,15 +573,29 @@ template <int PL> struct Sib {
         for (int p = 0; p < PL; p++)
             if (lane * PL + p == off) t[p] = tv;
     }
+    // Minimum of the node and the lowest voxel holding it: a pairwise min
+    // tree over the lane's PL times, the wave minimum by DPP, then one
+    // ballot per slot (voxel base + lane*PL + p; times past n are +inf, and
+    // an all-+inf node reports its first voxel).
     __device__ __forceinline__ void argmin(double &m, int &am, int lane, int n) const {
-        m = INFINITY;
-        am = INT_MAX;
+        double u[PL];
+#pragma unroll
+        for (int p = 0; p < PL; p++) u[p] = t[p];
+#pragma unroll
+        for (int st = 1; st < PL; st <<= 1)
+#pragma unroll
+            for (int i = 0; i + st < PL; i += 2 * st) u[i] = fmin(u[i], u[i + st]);
+        m = wave_min_dpp(u[0]);
+        int best = INT_MAX;
 #pragma unroll
         for (int p = 0; p < PL; p++) {
-            int c = base + lane * PL + p;
-            if (c < n) argmin2(m, am, t[p], c);
+            const unsigned long long h = __ballot(t[p] == m);
+            const int c = h ? (int)__builtin_ctzll(h) * PL + p : INT_MAX;
+            best = c < best ? c : best;
         }
-        wave_argmin_dpp(m, am);
+        am = base + best;
+        (void)lane;
+        (void)n;
     }
 };
 
@@ -592,16 +609,59 @@ struct TreeShape {
     __device__ __forceinline__ int cnt(int k) const { return __builtin_amdgcn_readlane(ln_l, k); }
 };
 
+// The top of the event tree also lives in registers: lane l holds node l
+// of level regl, the level just below the root (at most 64 nodes; level 1
+// when there are only one or two levels), and the root is uniform.  LDS
+// keeps every level (written back at the end of a launch); reads of these
+// two levels come from registers.
+struct TreeTop {
+    int regl, root;          // register level, LDS index of the root
+    double rt;               // lane l: node l of level regl (+inf past the count)
+    int rv;
+    double root_t;           // the root: the global minimum and its voxel
+    int root_v;
+};
+
+// Node idx of level regl changed from (old_m, old_am) to (m, am): update its
+// lane and the root, rescanning the register level only when the root's
+// minimum came from this node and got later.
+__device__ __forceinline__ void top_set(TreeTop &tp, Lds &L, int idx, double m, int am, double old_m, int old_am,
+                                        int lane) {
+    (void)old_m;
+    if (lane == idx) { tp.rt = m; tp.rv = am; }
+    double nm;
+    int nam;
+    if (tp.root_v == old_am && (m > tp.root_t || (m == tp.root_t && am != tp.root_v))) {
+        nm = tp.rt;
+        nam = tp.rv;
+        wave_argmin_dpp(nm, nam);
+    } else {
+        nm = tp.root_t;
+        nam = tp.root_v;
+        argmin2(nm, nam, m, am);
+    }
+    tp.root_t = nm;
+    tp.root_v = nam;
+    if (lane == 0) { L.lt[tp.root] = nm; L.lv[tp.root] = nam; }
+}
+
 // Repair the levels above a changed level-1 node value (idx -> (m, am)),
 // rescanning a node only when its stored minimum may have increased.
 template <int PL>
-__device__ void tree_fix_up(const TreeShape &a, Lds &L, int idx, double m, int am, double old_m, int old_am,
-                            int lane) {
-    for (int k = 1; k < a.nlev; k++) {
+__device__ void tree_fix_up(const TreeShape &a, Lds &L, TreeTop &tp, int idx, double m, int am, double old_m,
+                            int old_am, int lane) {
+    for (int k = 1; k < tp.regl; k++) {
         const int p = idx / WAVE;
         const int poff = a.off(k + 1) + p;
-        const double pm = L.lt[poff];
-        const int pam = L.lv[poff];
+        double pm;
+        int pam;
+        if (k + 1 == tp.regl) {
+            pm = rl_d(tp.rt, p);
+            pam = __builtin_amdgcn_readlane(tp.rv, p);
+        } else {
+            pm = L.lt[poff];
+            pam = L.lv[poff];
+        }
         double nm;
         int nam;
         if (pam == old_am && (m > pm || (m == pm && am != pam))) {
@@ -629,34 +689,53 @@ __device__ void tree_fix_up(const TreeShape &a, Lds &L, int idx, double m, int a
         am = nam;
         idx = p;
     }
+    top_set(tp, L, idx, m, am, old_m, old_am, lane);
+}
+
+// Level-1 node idx's stored value: registers when level 1 is the register
+// level, else LDS.
+__device__ __forceinline__ void level1_get(const TreeShape &a, const Lds &L, const TreeTop &tp, int idx, double &om,
+                                           int &oam) {
+    if (tp.regl == 1) {
+        om = rl_d(tp.rt, idx);
+        oam = __builtin_amdgcn_readlane(tp.rv, idx);
+    } else {
+        om = L.lt[a.off(1) + idx];
+        oam = L.lv[a.off(1) + idx];
+    }
 }
 
 // Voxel v gets time tv; its level-0 node is in sib (already holding tv).
+// known: v is the voxel the event loop selected, whose level-1 node's
+// minimum was (old_t, v) -- the root -- so the stored value is not read.
 template <int PL>
-__device__ void tree_update_rescan(const TreeShape &a, double *qtime, Lds &L, const Sib<PL> &sib, int v, double tv,
-                                   int lane) {
+__device__ void tree_update_rescan(const TreeShape &a, double *qtime, Lds &L, TreeTop &tp, const Sib<PL> &sib, int v,
+                                   double tv, int lane, bool known = false, double old_t = 0.0) {
     if (lane == 0) qtime[v] = tv;
     double m;
     int am;
     sib.argmin(m, am, lane, a.n);
     const int idx = v / (WAVE * PL);
     const int off = a.off(1) + idx;
-    const double om = L.lt[off];
-    const int oam = L.lv[off];
+    double om;
+    int oam;
+    if (known) { om = old_t; oam = v; }
+    else level1_get(a, L, tp, idx, om, oam);
     if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
-    if (!(m == om && am == oam)) tree_fix_up<PL>(a, L, idx, m, am, om, oam, lane);
+    if (!(m == om && am == oam)) tree_fix_up<PL>(a, L, tp, idx, m, am, om, oam, lane);
 }
 
 // Voxel v gets time tv without its level-0 node in registers: the node's
 // minimum is updated in place unless v was the node's argmin and got later,
 // in which case the node is reloaded and rescanned.
 template <int PL>
-__device__ void tree_update_point(const TreeShape &a, double *qtime, Lds &L, int v, double tv, int lane) {
+__device__ void tree_update_point(const TreeShape &a, double *qtime, Lds &L, TreeTop &tp, int v, double tv, int lane) {
     if (lane == 0) qtime[v] = tv;
     const int idx = v / (WAVE * PL);
     const int off = a.off(1) + idx;
-    const double om = L.lt[off];
-    const int oam = L.lv[off];
+    double om;
+    int oam;
+    level1_get(a, L, tp, idx, om, oam);
     double m;
     int am;
     if (oam == v && tv > om) {
@@ -675,7 +754,7 @@ __device__ void tree_update_point(const TreeShape &a, double *qtime, Lds &L, int
     }
     if (m == om && am == oam) return;
     if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
-    tree_fix_up<PL>(a, L, idx, m, am, om, oam, lane);
+    tree_fix_up<PL>(a, L, tp, idx, m, am, om, oam, lane);
 }
 
 // Snapshot k of one trajectory: its counts [n][S] and its species totals
@@ -759,6 +838,16 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
 #pragma unroll
     for (int k = 0; k < 10; k++)
         if (lane == k) { tsh.lo_l = a.lev_off[k]; tsh.ln_l = a.lev_n[k]; }
+    TreeTop tp;
+    {
+        tp.regl = a.nlev > 1 ? a.nlev - 1 : 1;
+        tp.root = root;
+        const int cnt = tsh.cnt(tp.regl), off = tsh.off(tp.regl);
+        tp.rt = lane < cnt ? L.lt[off + lane] : INFINITY;
+        tp.rv = lane < cnt ? L.lv[off + lane] : INT_MAX;
+        tp.root_t = L.lt[root];
+        tp.root_v = L.lv[root];
+    }
     LaneModel<MR> lm;
     {
         lm.kind = 0; lm.ra = 0; lm.rb = 0; lm.pc = 0; lm.k = 0.0; lm.mask = 0; lm.dep = 0;
@@ -839,8 +928,8 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
     while (tt <= end_time) {
         if (++guard > a.max_events) { err = SSASDPD_ERR_CAPACITY; break; }
         // ---- selectReaction: global minimum inside the window [L, U]
-        double t = L.lt[root];
-        int sv = L.lv[root];
+        double t = tp.root_t;
+        int sv = tp.root_v;
         bool found = false;
         for (int it = 0; it < 4096; it++) {
             if (sv != INT_MAX && t <= q.U) { found = true; break; }
@@ -948,7 +1037,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(tsh, qtime_, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(tsh, qtime_, L, tp, sib, sv, ntv, lane, true, tt);
         } else {
             // ---- diffusion (simulate_rdme.cpp:312-445)
             event = 1;
@@ -1140,7 +1229,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(tsh, qtime_, L, sib, sv, ntv, lane);
+            tree_update_rescan<PL>(tsh, qtime_, L, tp, sib, sv, ntv, lane, true, tt);
             const double dest_tot = srd + sdd;
             const double dold = self ? ntv : qtd;
             double ntd;
@@ -1153,9 +1242,9 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             }
             if (sib.holds(dest)) {
                 sib.set(dest, ntd, lane);
-                tree_update_rescan<PL>(tsh, qtime_, L, sib, dest, ntd, lane);
+                tree_update_rescan<PL>(tsh, qtime_, L, tp, sib, dest, ntd, lane);
             } else {
-                tree_update_point<PL>(tsh, qtime_, L, dest, ntd, lane);
+                tree_update_point<PL>(tsh, qtime_, L, tp, dest, ntd, lane);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
