@@ -1,0 +1,82 @@
+"""Statistical parity of the seeded GPU stream: ensembles run on the GPU with
+its counter-based generator against ensembles of the reference algorithm
+with the reference's own generator (the oracle, std::mt19937_64 seeded like
+the generated program, propensity_file_template.cpp:127).  Per output time:
+two-sample Kolmogorov-Smirnov on the species totals, the difference of
+means inside its 95% band, the variance ratio inside its F band; and a
+spatial statistic (the share of molecules in the left half of the domain).
+Seeds are fixed, so every run of these tests gives the same p-values."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from spatialpy_amd import workloads
+from spatialpy_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu   # (the helpers below are also used by test_statistics_cpu.py)
+
+N_TRAJ = 400
+ALPHA = 1e-3      # per comparison; about 30 comparisons per model
+
+
+def _ensembles(sa, oracle, seed0):
+    eng = Engine(sa)
+    seeds = [seed0 + i for i in range(N_TRAJ)]
+    gpu = eng.run_batch(seeds, fields=("xx",))["xx"].astype(np.int64)        # [B][K][n][S]
+    o = oracle.Oracle(sa)
+    ref = np.stack([o.run(seed=10_000 + seed0 + i)["xx"].astype(np.int64) for i in range(N_TRAJ)])
+    assert gpu.shape == ref.shape
+    return gpu, ref
+
+
+def _compare(a, b, what):
+    """a, b: samples of one statistic from the two ensembles."""
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    if a.std() == 0 and b.std() == 0:
+        assert a.mean() == b.mean(), what
+        return
+    ks = stats.ks_2samp(a, b)
+    assert ks.pvalue > ALPHA, (what, "KS", ks)
+    se = np.sqrt(a.var(ddof=1) / a.size + b.var(ddof=1) / b.size)
+    z = abs(a.mean() - b.mean()) / se
+    assert z < stats.norm.ppf(1 - ALPHA / 2), (what, "mean", a.mean(), b.mean(), z)
+    f = a.var(ddof=1) / b.var(ddof=1)
+    lo = stats.f.ppf(ALPHA / 2, a.size - 1, b.size - 1)
+    hi = stats.f.ppf(1 - ALPHA / 2, a.size - 1, b.size - 1)
+    assert lo < f < hi, (what, "variance ratio", f)
+
+
+def _check(sa, gpu, ref, name):
+    left = sa.x[:, 0] < 0.5 * (sa.x[:, 0].min() + sa.x[:, 0].max())
+    K, S = gpu.shape[1], gpu.shape[3]
+    for k in range(1, K):
+        for s in range(S):
+            tg, tr = gpu[:, k, :, s].sum(1), ref[:, k, :, s].sum(1)
+            _compare(tg, tr, f"{name} total species {s} snapshot {k}")
+            if tg.min() > 0 and tr.min() > 0:
+                _compare(gpu[:, k, left, s].sum(1) / tg, ref[:, k, left, s].sum(1) / tr,
+                         f"{name} left-half share species {s} snapshot {k}")
+
+
+def test_birth_death_ensemble_matches_reference_generator(oracle):
+    """The reference's biased-selection birth-death model (config 1, t = 0..4)."""
+    sa = workloads.birth_death_2d(seed=1, t_end=4, num_points=5)
+    gpu, ref = _ensembles(sa, oracle, 100)
+    _check(sa, gpu, ref, "birth-death")
+
+
+def test_reversible_binding_ensemble_matches_reference_generator(oracle):
+    """A + B <-> C on a static 3D lattice (config 2 at 8^3 voxels)."""
+    sa = workloads.static_rd_3d(n_side=8, nA=800, nB=800, t_end=3, num_points=4, seed=2)
+    gpu, ref = _ensembles(sa, oracle, 500)
+    _check(sa, gpu, ref, "A+B<->C")
+
+
+def test_seeded_stream_is_not_the_replay_stream(oracle):
+    """Sanity: the two generators give different trajectories (the
+    comparison above is between independent ensembles)."""
+    sa = workloads.birth_death_2d(seed=1, t_end=2, num_points=3)
+    g = Engine(sa).run(seed=7, fields=("xx",))["xx"]
+    r = oracle.Oracle(sa).run(seed=7)["xx"]
+    assert not np.array_equal(g, r)
