@@ -9,6 +9,11 @@ ssa_sdpd-c-simulation-engine/src/simulate_threads.cpp:232-281).
 Prints ONE JSON line (metric particle-steps/s, higher is better), with the
 SSA event rate, the roofline of the dominant HBM-bound kernels and the
 reference CPU engine timed on this host's cores beside it.
+
+Two more workloads print the same kind of line: ``--workload static3d``
+(the 40^3 static A+B<->C case: SSA events/s of a seed batch, with one
+trajectory beside it) and ``--workload ensemble`` (uneven gene-network
+models in one persistent launch: trajectories/s).
 """
 import argparse
 import json
@@ -117,15 +122,216 @@ def surrogate_timing(eng, sa, plain_pair_ms):
                     "(tests/test_gpu_mpn.py), not here"}
 
 
+def ssa_bytes_per_event(S, R, E):
+    """Algorithmic bytes of one event: a reaction reads and writes the
+    selected voxel's state (type, degree, vol, queue time: 28 B; counts 4 S;
+    Ddiag 8 S; rates 8 R; srrate/sdrate 16) and writes back its counts, rates
+    and sums; a diffusion does that for the source and the destination and
+    reads the source's neighbour row (int32 index + FP64 D_ij)."""
+    vox = 28 + 4 * S + 8 * S + 8 * R + 16 + 4 * S + 8 * R + 16
+    return vox, 2 * vox + 12 * E
+
+
+def mean_degree(x, h, samples=2000, seed=0):
+    """Mean neighbour count within h (self excluded) over a sample of points."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(x.shape[0], size=min(samples, x.shape[0]), replace=False)
+    cnt = cKDTree(x).query_ball_point(x[idx], r=h, return_length=True)
+    return float(np.mean(cnt) - 1.0)
+
+
+def ssa_roofline(S, R, E, n_re, n_di, gpu_ms, kernel):
+    b_react, b_diff = ssa_bytes_per_event(S, R, E)
+    by = b_react * n_re + b_diff * n_di
+    gbs = by / (gpu_ms * 1e-3) / 1e9 if gpu_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(gbs, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 7), "traffic": None,
+            "kernel": kernel, "algorithmic_bytes_per_event": {"reaction": b_react, "diffusion": b_diff},
+            "mean_degree": round(E, 2), "events": int(n_re + n_di),
+            "us_per_event": round(1e3 * gpu_ms / max(n_re + n_di, 1), 3)}
+
+
+def reference_timing(sa, steps):
+    """The reference engine (oracle/_ref/ref_harness_O2, built from the
+    reference sources) on one model for `steps` steps, with min(nproc, 16)
+    threads: (seconds, reactions + diffusions, threads), or None when the
+    build is absent."""
+    threads = min(os.cpu_count() or 1, 16)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_O2")
+    if not os.path.exists(exe):
+        return None
+    from spatialpy_amd import lowering
+    with tempfile.TemporaryDirectory() as d:
+        lowering.write_harness_dir(sa, d, seed=1, threads=threads, run_steps=steps)
+        r = subprocess.run([exe, d], capture_output=True, text=True, timeout=600)
+    line = [l for l in r.stdout.splitlines() if l.startswith("REF_TIMING")]
+    if r.returncode != 0 or not line:
+        return None
+    kv = dict(tok.split("=") for tok in line[0].split()[1:])
+    return float(kv["seconds"]), int(kv["reactions"]) + int(kv["diffusions"]), threads
+
+
+def oracle_timing(sa, steps):
+    """The repository's FP64 restatement, one thread, when the reference
+    build is absent: (seconds, events, 1)."""
+    from oracle import oracle_py
+    o = oracle_py.Oracle(sa)
+    o.reset(seed=1)
+    t = time.perf_counter()
+    o.advance(steps)
+    sec = time.perf_counter() - t
+    st = o.stats()
+    return sec, int(st.reactions + st.diffusions), 1
+
+
+def bench_static3d(args):
+    """The static 3D reaction-diffusion case (40^3 = 64,000 fixed voxels,
+    A+B<->C, 1e5 A + 1e5 B): one trajectory, then a batch of seeds of the
+    same model (one event-loop workgroup per trajectory), over the first
+    `steps` unit time steps of its t = 0..100 run."""
+    from spatialpy_amd import workloads
+    from spatialpy_amd.engine import Engine
+    steps = args.steps if args.steps is not None else 2
+    warm = args.warmup if args.warmup is not None else 0
+    B = args.batch
+    sa = workloads.static_rd_3d(t_end=warm + steps, num_points=warm + steps + 1)
+    E = mean_degree(sa.x, sa.h)
+    eng = Engine(sa)
+    eng.reset(seed=12345)
+    if warm:
+        eng.advance(warm)
+    st0, ph0 = eng.stats(), eng.phase_times()
+    t0 = time.perf_counter()
+    eng.advance(steps)
+    sec = time.perf_counter() - t0
+    st1, ph1 = eng.stats(), eng.phase_times()
+    n_re, n_di = st1.reactions - st0.reactions, st1.diffusions - st0.diffusions
+    ssa_ms = ph1["ssa"][0] - ph0["ssa"][0]
+    single = {"events": int(n_re + n_di), "seconds": round(sec, 4), "events_per_s": (n_re + n_di) / sec,
+              "ssa_gpu_ms": round(ssa_ms, 3)}
+    roof = ssa_roofline(sa.n_species, sa.n_reactions, E, n_re, n_di, ssa_ms,
+                        "k_ssa (event loop; latency-bound, one wavefront per trajectory)")
+    eng.close()
+    # the batch: B seeded trajectories of the whole window (t = 0..warm+steps)
+    beng = Engine(sa)
+    t0 = time.perf_counter()
+    beng.run_batch(list(range(1, B + 1)), fields=("xx",))
+    bsec = time.perf_counter() - t0
+    bst = beng.stats()
+    bph = beng.phase_times()
+    bev = bst.reactions + bst.diffusions
+    batch = {"trajectories": B, "sim_time": float((warm + steps) * sa.dt), "events": int(bev),
+             "seconds": round(bsec, 4), "events_per_s": bev / bsec, "trajectories_per_s": B / bsec,
+             "ssa_gpu_ms": round(bph["ssa"][0], 3),
+             "note": "wall time includes copying every trajectory's count snapshots to the host"}
+    beng.close()
+    out = {
+        "metric": "SSA events/s", "value": bev / bsec, "unit": "events/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": 1e3 * sec / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"static 3D reaction-diffusion, 40^3 = {sa.n} fixed voxels on the unit cube, "
+                               f"A+B->C k=1e-3, C->A+B k=0.1, D = 0.01/0.01/0.001, 1e5 A + 1e5 B scattered, "
+                               f"dt=1, t = 0..{warm + steps}; value: {B} seeded trajectories in one batch",
+                   "voxels": sa.n, "batch": B},
+        "single_trajectory": single,
+        "batch": batch,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        r = reference_timing(sa, warm + steps)
+        kind = "reference"
+        if r is None:
+            r, kind = oracle_timing(sa, warm + steps), "port"
+        csec, cev, thr = r
+        out["cpu_baseline"] = {"value": cev / csec, "unit": "events/s", "cores": thr, "kind": kind,
+                               "sample": f"one trajectory of the same model, t = 0..{warm + steps}: {cev} events "
+                                         f"in {csec:.2f} s (the reference's event loop is serial)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["single_trajectory"]["speedup_vs_cpu"] = single["events_per_s"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+def bench_ensemble(args):
+    """Uneven gene-network models (jittered 3D lattices, N ~ U[8k, 64k], own
+    rates and initial counts each) in one persistent launch, after the
+    event-tree width autotuner; t = 0..t_end."""
+    from spatialpy_amd import workloads
+    from spatialpy_amd.ensemble import Ensemble
+    M, t_end = args.models, args.t_end
+    rng = np.random.default_rng(0)
+    sizes = rng.integers(8000, 64001, size=M)
+    t0 = time.perf_counter()
+    models = [workloads.gene_network_3d(n_target=int(nv), seed=1000 + i, t_end=t_end) for i, nv in enumerate(sizes)]
+    build_s = time.perf_counter() - t0
+    print(f"built {M} models in {build_s:.1f} s", file=sys.stderr, flush=True)
+    E = float(np.mean([mean_degree(m.x, m.h, samples=500, seed=i) for i, m in enumerate(models[:16])]))
+    ens = Ensemble(models)
+    seeds = list(range(1, M + 1))
+    width, probe = ens.autotune(seeds, probe_steps=1)
+    print(f"autotuned event-tree width {width}: {probe}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ens.run(seeds)
+    sec = time.perf_counter() - t0
+    st = ens.stats()
+    ev = st.reactions + st.diffusions
+    S, R = models[0].n_species, models[0].n_reactions
+    roof = ssa_roofline(S, R, E, st.reactions, st.diffusions, st.ms_ssa,
+                        "k_ssa_multi (all models' event loops in one launch; latency-bound)")
+    roof["note"] = "achieved is the sum over concurrent trajectories of their algorithmic bytes per event"
+    voxels = int(sum(m.n for m in models))
+    out = {
+        "metric": "trajectories/s", "value": M / sec, "unit": "trajectories/s", "n_gpus": 1,
+        "steps": int(models[0].nt), "warmup": 0, "ms_per_step": 1e3 * sec / max(int(models[0].nt), 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{M} uneven static gene-network models (5 species, 6 mass-action reactions, "
+                               f"jittered 3D lattices with N ~ U[8000, 64000], rates log-U[1e-3, 10], "
+                               f"D log-U[1e-4, 1e-2], Poisson(50) counts), one trajectory each, t = 0..{t_end}, "
+                               f"one persistent launch",
+                   "models": M, "voxels": voxels, "t_end": t_end, "tree_width": width},
+        "ssa_events_per_s": ev / sec,
+        "ssa_events": int(ev),
+        "autotune_ms": probe,
+        "host_model_build_s": round(build_s, 2),
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        k = min(args.cpu_models, M)
+        tot_s, tot_ev, thr, kind = 0.0, 0, 1, "reference"
+        for m in models[:k]:
+            r = reference_timing(m, int(m.nt))
+            if r is None:
+                r, kind = oracle_timing(m, int(m.nt)), "port"
+            tot_s += r[0]
+            tot_ev += r[1]
+            thr = r[2]
+        out["cpu_baseline"] = {"value": k / tot_s, "unit": "trajectories/s", "cores": thr, "kind": kind,
+                               "sample": f"the first {k} of the same models run one after another, as the "
+                                         f"reference runs trajectories: {tot_ev} events in {tot_s:.2f} s",
+                               "events_per_s": tot_ev / tot_s}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("cylinder", "static3d", "ensemble"), default="cylinder",
+                    help="cylinder: the 1M-particle moving SDPD+sSSA run (the headline line); static3d: the "
+                         "64k-voxel static A+B<->C case, one trajectory and a seed batch; ensemble: uneven "
+                         "gene-network models in one launch")
     # the open cylinder at h = 1.3 x spacing, c0 = 20, dt = 2.5e-4 turns
     # non-finite at step 21 in the reference engine itself (and identically
     # here), so the default window stays inside steps 0..17
-    ap.add_argument("--steps", type=int, default=15)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (cylinder 15, static3d 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (cylinder 2, static3d 0)")
     ap.add_argument("--n", type=int, default=1_000_000, help="particles in the cylinder")
+    ap.add_argument("--batch", type=int, default=256, help="static3d: trajectories in the seed batch")
+    ap.add_argument("--models", type=int, default=256, help="ensemble: number of models")
+    ap.add_argument("--t-end", type=int, default=10, help="ensemble: simulated time (unit steps)")
+    ap.add_argument("--cpu-models", type=int, default=8, help="ensemble: models in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
@@ -145,13 +351,20 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nb, "dispatches": 3}))
         return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build has no "
+              "multi-GPU path (domain slabs with halo exchange, and ensembles split by trajectory, are "
+              "design only)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "static3d":
+        return bench_static3d(args)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    args.steps = 15 if args.steps is None else args.steps
+    args.warmup = 2 if args.warmup is None else args.warmup
     if args.warmup + args.steps > 20:
         print("warning: this workload turns non-finite at step 21 (also in the reference engine); "
               "keep warmup + steps <= 20", file=sys.stderr)
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
 
     from spatialpy_amd.engine import Engine
     from spatialpy_amd import _abi
@@ -191,8 +404,7 @@ def main():
     # bytes are the voxel states an event reads and writes
     ssa_ms = dph["ssa"][0]
     S_, R_ = sa.n_species, sa.n_reactions
-    vox = 28 + 4 * S_ + 8 * S_ + 8 * R_ + 16 + 4 * S_ + 8 * R_ + 16
-    b_react, b_diff = vox, 2 * vox + 12 * E
+    b_react, b_diff = ssa_bytes_per_event(S_, R_, E)
     n_re = st1.reactions - st0.reactions
     n_di = st1.diffusions - st0.diffusions
     ssa_bytes = b_react * n_re + b_diff * n_di
