@@ -234,7 +234,12 @@ __device__ bool set_bins(QueueState &q, double newL, double psum, long long acti
     q.L = newL;
     if (q.L > q.endTime) return false;
     double w = 16.0 / psum;
-    unsigned long long maxb = (unsigned long long)((double)20 * sqrt((double)(int)active));
+    // The empty asm keeps the square root here: without it the compiler
+    // hoists it out of the rarely taken window rebuild into every event of
+    // the event loop, since `active` does not change inside the rebuild.
+    double act = (double)(int)active;
+    asm volatile("" : "+v"(act));
+    unsigned long long maxb = (unsigned long long)((double)20 * sqrt(act));
     unsigned long long endb = ULLONG_MAX, nb;
     if (q.endTime < DBL_MAX) {
         endb = (unsigned long long)ceil((q.endTime - q.L) / w);
@@ -443,6 +448,9 @@ __device__ __forceinline__ int dpp_i(int x) {
 template <int CTRL, int RMASK>
 __device__ __forceinline__ double dpp_d(double x) {
     int lo = __double2loint(x), hi = __double2hiint(x);
+    if (RMASK == 0xF)   // every lane is written: no copy of the old value
+        return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false),
+                                __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false));
     return __hiloint2double(dpp_i<CTRL, RMASK>(hi), dpp_i<CTRL, RMASK>(lo));
 }
 template <int CTRL, int RMASK>
@@ -580,9 +588,10 @@ template <int PL> struct Sib {
             if (lane * PL + p == off) t[p] = tv;
     }
     // Minimum of the node and the lowest voxel holding it: a pairwise min
-    // tree over the lane's PL times, the wave minimum by DPP, then one
-    // ballot per slot (voxel base + lane*PL + p; times past n are +inf, and
-    // an all-+inf node reports its first voxel).
+    // tree over the lane's PL times, the wave minimum by DPP, then each
+    // lane's first slot holding it and one ballot for the first such lane
+    // (voxel base + lane*PL + p; times past n are +inf, and an all-+inf node
+    // reports its first voxel).
     __device__ __forceinline__ void argmin(double &m, int &am, int lane, int n) const {
         double u[PL];
 #pragma unroll
@@ -592,14 +601,13 @@ template <int PL> struct Sib {
 #pragma unroll
             for (int i = 0; i + st < PL; i += 2 * st) u[i] = fmin(u[i], u[i + st]);
         m = wave_min_dpp(u[0]);
-        int best = INT_MAX;
+        int slot = PL;
 #pragma unroll
-        for (int p = 0; p < PL; p++) {
-            const unsigned long long h = __ballot(t[p] == m);
-            const int c = h ? (int)__builtin_ctzll(h) * PL + p : INT_MAX;
-            best = c < best ? c : best;
-        }
-        am = base + best;
+        for (int p = PL - 1; p >= 0; p--)
+            if (t[p] == m) slot = p;
+        const unsigned long long h = __ballot(slot < PL);
+        const int w = h ? (int)__builtin_ctzll(h) : 0;
+        am = base + w * PL + (h ? __builtin_amdgcn_readlane(slot, w) : 0);
         (void)lane;
         (void)n;
     }
