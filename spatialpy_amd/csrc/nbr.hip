@@ -584,6 +584,16 @@ void kd_build(hipStream_t st, const double *pts, int n, int dim, KdDev &kd, void
 
 // ---------------------------------------------------------------------------
 // Fixed-radius search.
+
+// Each lane of the search builds and sorts its list in FR_SLOTS LDS slots
+// (slot-major, one column per lane of a one-wavefront block) and writes it
+// out once.  A list that outgrows them moves to the output arrays (entries
+// in visit order, then the later hits) and is sorted there.  A few slots keep
+// enough wavefronts resident for the gathers: most lists on a moving lattice
+// at h = 1.3 spacing hold 6 neighbours.
+constexpr int FR_SLOTS = 12;
+constexpr size_t FR_LDS_BYTES = (size_t)FR_SLOTS * WAVE * (sizeof(int) + sizeof(double));
+
 struct KdView {
     const int *parent, *depth, *cd, *is_hi, *leaf_of;
     const double *cv;
@@ -606,16 +616,46 @@ __device__ bool kd_before(const KdView &kd, int a, int b, const double *q) {
     return kd.is_hi[na] == near_hi;
 }
 
-__global__ void __launch_bounds__(256)
+// Insertion sort of a list of cnt entries by (r2, ANN visit order); I and D
+// return references to entry k's index and r2.
+template <class IA, class DA>
+__device__ __forceinline__ void sort_list(IA I, DA D, int cnt, const KdView &kd, const double *qp) {
+    for (int i = 1; i < cnt; i++) {
+        int ji = I(i);
+        double ri = D(i);
+        int k = i - 1;
+        while (k >= 0) {
+            double rk = D(k);
+            int jk = I(k);
+            bool after = rk > ri || (rk == ri && kd_before(kd, ji, jk, qp));
+            if (!after) break;
+            I(k + 1) = jk;
+            D(k + 1) = rk;
+            k--;
+        }
+        I(k + 1) = ji;
+        D(k + 1) = ri;
+    }
+}
+
+__global__ void __launch_bounds__(WAVE)
 k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts, const int *__restrict__ sorted,
             const int *__restrict__ cell_start, int n, int dim, double h, Grid g, KdView kd, int W,
             int *__restrict__ out_idx, double *__restrict__ out_r2, int *__restrict__ deg, DevErr *err,
             const int *__restrict__ order) {
-    const int chunk = xcd_chunk((m + 255) / 256);
+    __shared__ int l_idx[FR_SLOTS * WAVE];
+    __shared__ double l_r2[FR_SLOTS * WAVE];
+    const int tl = (int)threadIdx.x;
+    const int chunk = xcd_chunk((m + WAVE - 1) / WAVE);
     if (chunk < 0) return;
-    int t = chunk * blockDim.x + threadIdx.x;
+    int t = chunk * WAVE + tl;
     if (t >= m) return;
     int qi = order ? order[t] : t;
+    auto LI = [&](int k) -> int & { return l_idx[k * WAVE + tl]; };
+    auto LD = [&](int k) -> double & { return l_r2[k * WAVE + tl]; };
+    auto GI = [&](int k) -> int & { return out_idx[(size_t)k * m + qi]; };
+    auto GD = [&](int k) -> double & { return out_r2[(size_t)k * m + qi]; };
+    const int L = W < FR_SLOTS ? W : FR_SLOTS;   // LDS slots in use
     double qp[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < dim; d++) qp[d] = q[(size_t)d * m + qi];
     int c[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -639,9 +679,14 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                 if (dim > 1) { double t1 = qp[1] - spts[(size_t)n + p]; dist = dist + t1 * t1; }
                 if (dim > 2) { double t2 = qp[2] - spts[2 * (size_t)n + p]; dist = dist + t2 * t2; }
                 if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) {
-                    if (cnt < W) {
-                        out_idx[(size_t)cnt * m + qi] = sorted[p];
-                        out_r2[(size_t)cnt * m + qi] = dist;
+                    const int j = sorted[p];
+                    if (cnt < L) {
+                        LI(cnt) = j;
+                        LD(cnt) = dist;
+                    } else {
+                        if (cnt == L)   // the list moves to the output arrays
+                            for (int k = 0; k < L; k++) { GI(k) = LI(k); GD(k) = LD(k); }
+                        if (cnt < W) { GI(cnt) = j; GD(cnt) = dist; }
                     }
                     cnt++;
                 }
@@ -652,22 +697,11 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
         deg[qi] = 0;
     } else {
         deg[qi] = cnt;
-        // insertion sort by (r2, ANN visit order)
-        for (int i = 1; i < cnt; i++) {
-            int ji = out_idx[(size_t)i * m + qi];
-            double ri = out_r2[(size_t)i * m + qi];
-            int k = i - 1;
-            while (k >= 0) {
-                double rk = out_r2[(size_t)k * m + qi];
-                int jk = out_idx[(size_t)k * m + qi];
-                bool after = rk > ri || (rk == ri && kd_before(kd, ji, jk, qp));
-                if (!after) break;
-                out_idx[(size_t)(k + 1) * m + qi] = jk;
-                out_r2[(size_t)(k + 1) * m + qi] = rk;
-                k--;
-            }
-            out_idx[(size_t)(k + 1) * m + qi] = ji;
-            out_r2[(size_t)(k + 1) * m + qi] = ri;
+        if (cnt <= L) {
+            sort_list(LI, LD, cnt, kd, qp);
+            for (int k = 0; k < cnt; k++) { GI(k) = LI(k); GD(k) = LD(k); }
+        } else {
+            sort_list(GI, GD, cnt, kd, qp);
         }
     }
     atomicMax(&err->max_deg, cnt);
@@ -681,8 +715,8 @@ void fr_search(hipStream_t s, const double *q, int m, const double *spts, const 
         v.parent = kd->parent; v.depth = kd->depth; v.cd = kd->cd; v.is_hi = kd->is_hi;
         v.leaf_of = kd->leaf_of; v.cv = kd->cv; v.valid = 1;
     }
-    hipLaunchKernelGGL(k_fr_search, xcd_grid((m + 255) / 256), dim3(256), 0, s, q, m, spts, sorted, cell_start, n,
-                       dim, h, g, v, W, out_idx, out_r2, deg, err, order);
+    hipLaunchKernelGGL(k_fr_search, xcd_grid((m + WAVE - 1) / WAVE), dim3(WAVE), 0, s, q, m, spts, sorted, cell_start,
+                       n, dim, h, g, v, W, out_idx, out_r2, deg, err, order);
 }
 
 // ---------------------------------------------------------------------------
