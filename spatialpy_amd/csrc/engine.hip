@@ -92,6 +92,7 @@ struct ssasdpd_engine {
     int W = 0;
     int *nidx = nullptr, *deg = nullptr;
     double *nr2 = nullptr, *nr = nullptr, *Dij = nullptr;
+    EdgeRec *erow = nullptr;   // the rows voxel-major, for the event loop
     // binning / tree
     double *xtree = nullptr, *spts = nullptr, *d_bbox = nullptr;
     unsigned long long *d_bbtmp = nullptr;
@@ -183,7 +184,7 @@ struct ssasdpd_engine {
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
         dfree(rho); dfree(old_rho); dfree(Frho); dfree(mass); dfree(nu); dfree(bvf);
         dfree(type); dfree(solid); dfree(C); dfree(Q);
-        dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(vol);
+        dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(erow); dfree(vol);
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
         dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
         dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
@@ -206,7 +207,7 @@ struct ssasdpd_engine {
     }
 
     void alloc_lists(int w) {
-        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype);
+        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(erow);
         W = w;
         size_t cap = (size_t)W * n;
         nidx = dalloc<int>(cap);
@@ -214,6 +215,7 @@ struct ssasdpd_engine {
         nr2 = dalloc<double>(cap);
         nr = dalloc<double>(cap);
         Dij = dalloc<double>(cap);
+        erow = dalloc<EdgeRec>(cap);
     }
 
     // options set before setup (ensembles): a common event-tree width and a
@@ -544,7 +546,7 @@ struct ssasdpd_engine {
             if (!e.overflow) break;
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
-        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr, type, ntype);
+        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr, type, ntype, erow);
         ph_end(1);
         lists_built = true;
     }
@@ -568,6 +570,8 @@ struct ssasdpd_engine {
         a.mass = mass; a.rho = rho; a.type = type; a.xx = xx; a.rrate = rrate; a.srrate = srrate; a.sdrate = sdrate;
         a.Ddiag = Ddiag; a.Dmat = Dmat; a.rx = rx; a.jcN = jcN; a.irN = irN; a.prN = prN; a.jcG = jcG; a.irG = irG;
         a.deg = deg; a.nidx = nidx; a.ntype = ntype; a.Dij = Dij; a.vol = vol;
+        a.erow = erow;
+        a.row_w = std::max(1, std::min(W, max_deg_seen));
         a.qtime = qtime; a.tmin = tmin; a.targ = targ; a.nlev = nlev; a.pl = pl; a.tree_nodes = tree_nodes;
         a.lds_bytes = lds_bytes;
         for (int l = 0; l < 10; l++) { a.lev_off[l] = lev_off[l]; a.lev_n[l] = lev_n[l]; }

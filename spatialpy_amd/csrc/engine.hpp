@@ -122,10 +122,20 @@ size_t kd_work_bytes(int n);
 size_t scan_tmp_bytes(long long n);
 void exclusive_scan_i32(hipStream_t s, const int *in, int *out, long long n, void *tmp);
 
-// D_ij and r from r2 (add_to_neighbor_list, ENG/src/particle.cpp:150-210).
+// One edge of a voxel's neighbour row as the event loop reads it: a row is
+// contiguous (erow[i*W + k]), so one event's row load touches a few cache
+// lines instead of one per slot of the neighbour-major ELL arrays.
+struct EdgeRec {
+    int j;        // neighbour
+    int type;     // its type
+    double D;     // D_ij
+};
+
+// D_ij and r from r2 (add_to_neighbor_list, ENG/src/particle.cpp:150-210),
+// plus the voxel-major rows erow.
 void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2,
                   double *r, double *Dij, const double *mass, const double *rho, double h,
-                  DevErr *err, const int *type, int *ntype);
+                  DevErr *err, const int *type, int *ntype, EdgeRec *erow);
 
 // SDPD substeps.
 struct SdpdArgs {
@@ -170,6 +180,8 @@ struct RdmeArgs {
     const int *jcN, *irN, *prN, *jcG, *irG;
     const int *deg, *nidx, *ntype;
     const double *Dij;
+    const EdgeRec *erow;     // the same rows voxel-major (event loop)
+    int row_w;               // no row is longer (<= W)
     double *vol;             // mass/rho per voxel, set by the rates kernel
     // queue: level 0 = firing times of the voxels; level k >= 1 holds the
     // (time, voxel) minimum of its children: F0 = 64*pl voxels per level-1

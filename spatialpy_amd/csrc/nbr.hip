@@ -723,7 +723,7 @@ void fr_search(hipStream_t s, const double *q, int m, const double *spts, const 
 // r = sqrt(r2) and D_ij per edge (add_to_neighbor_list, particle.cpp:160-207).
 __global__ void k_finish(int n, int W, const int *deg, const int *idx, const double *r2, double *r, double *Dij,
                          const double *mass, const double *rho, double h, DevErr *err, const int *type,
-                         int *ntype) {
+                         int *ntype, EdgeRec *erow) {
     const int chunk = xcd_chunk((n + 255) / 256);
     if (chunk < 0) return;
     int i = chunk * blockDim.x + threadIdx.x;
@@ -745,17 +745,19 @@ __global__ void k_finish(int n, int W, const int *deg, const int *idx, const dou
             atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
             atomicMin(&err->particle, i);
         }
+        const int tj = type[j];
         r[e] = rr;
         Dij[e] = D;
-        ntype[e] = type[j];
+        ntype[e] = tj;
+        erow[(size_t)i * W + k] = EdgeRec{j, tj, D};
     }
 }
 
 void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2, double *r,
                   double *Dij, const double *mass, const double *rho, double h, DevErr *err, const int *type,
-                  int *ntype) {
+                  int *ntype, EdgeRec *erow) {
     hipLaunchKernelGGL(k_finish, xcd_grid((n + 255) / 256), dim3(256), 0, s, n, W, deg, idx, r2, r, Dij, mass, rho, h,
-                       err, type, ntype);
+                       err, type, ntype, erow);
 }
 
 }  // namespace ssa

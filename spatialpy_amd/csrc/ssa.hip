@@ -843,7 +843,8 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         __syncthreads();
     }
     const int root = a.lev_off[a.nlev];
-    const int W = a.W;
+    const int W = a.W;     // row stride of erow
+    const int RW = a.row_w;   // lanes that load a row (no row is longer)
     TreeShape tsh;
     tsh.n = n;
     tsh.nlev = a.nlev;
@@ -963,22 +964,22 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         const int tsv = a.type[sv];
         const int dg_l = a.deg[sv];
         // lane s: count of species s; lane j: rate of reaction j; lanes past
-        // S / R / W read a valid element and drop it
+        // S / R / the longest row read a valid element and drop it
         const int ls = lane < S ? lane : S - 1, lr = lane < R ? lane : (R > 0 ? R - 1 : 0);
-        const int lw = lane < W ? lane : W - 1;
+        const int lw = lane < RW ? lane : RW - 1;
         const uint32_t xs_raw = xx_[(size_t)sv * S + ls];
         const double Dd_raw = a.Ddiag[(size_t)sv * S + ls];
         const double rr_raw = R > 0 ? rrate_[(size_t)sv * R + lr] : 0.0;
-        const size_t e_l = (size_t)lw * n + sv;
-        const int nbj_raw = a.nidx[e_l];
-        const double nbD_raw = a.Dij[e_l];
-        const int nbt_raw = a.ntype[e_l];
+        const EdgeRec er = a.erow[(size_t)sv * W + lw];
+        const int nbj_raw = er.j;
+        const double nbD_raw = er.D;
+        const int nbt_raw = er.type;
         uint32_t xs = s_lane ? xs_raw : 0u;
         const double Dd = s_lane ? Dd_raw : 0.0;
         double rr = r_lane ? rr_raw : 0.0;
-        const int nb_j = lane < W ? nbj_raw : -1;
-        const double nb_D = lane < W ? nbD_raw : 0.0;
-        const int nb_t = (lane < W && T > 1) ? nbt_raw : 1;
+        const int nb_j = lane < RW ? nbj_raw : -1;
+        const double nb_D = lane < RW ? nbD_raw : 0.0;
+        const int nb_t = (lane < RW && T > 1) ? nbt_raw : 1;
         double totrate = sr + sd;
         const double rand1 = rng.uniform(lane);
         if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
@@ -1134,9 +1135,9 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                         const int k = c0 + lane;
                         double cl = 0.0;
                         if (k < dg) {
-                            const size_t e = (size_t)k * n + sv;
-                            const int tk = T > 1 ? a.ntype[e] : 1;
-                            cl = a.Dij[e] * L.Dmat[spec * T + (tk - 1)];
+                            const EdgeRec ek = a.erow[(size_t)sv * W + k];
+                            const int tk = T > 1 ? ek.type : 1;
+                            cl = ek.D * L.Dmat[spec * T + (tk - 1)];
                         }
                         const int mm = dg - c0 < WAVE ? dg - c0 : WAVE;
                         for (int q2 = 0; q2 < mm; q2++) {
@@ -1170,7 +1171,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             } else {
                 if (kk >= 0)
                     dest = dg <= WAVE ? __builtin_amdgcn_readlane(nb_j, kk)
-                                      : __builtin_amdgcn_readfirstlane(a.nidx[(size_t)kk * n + sv]);
+                                      : __builtin_amdgcn_readfirstlane(a.erow[(size_t)sv * W + kk].j);
                 const size_t j = (size_t)dest;
                 xd = s_lane ? xx_[j * S + lane] : 0u;
                 rrd = r_lane ? rrate_[j * R + lane] : 0.0;
