@@ -279,18 +279,22 @@ struct ssasdpd_engine {
         mpn_apply(st, n, dim, nn->out_buf(), solid, g, nn->acc_scale, Fout);
     }
 
+    // Level-1 nodes held in the event loop's LDS (with the levels above:
+    // about 48 KiB at the limit).  The narrowest width that fits is the
+    // fastest: each event loads one level-0 node of 64*pl firing times.
+    static constexpr int L1_MAX = 4096;
     static int auto_pl(int nv) {
         int p = 1;
-        while (p < 32 && (nv + 64 * p - 1) / (64 * p) > 3072) p *= 2;
+        while (p < 32 && (nv + 64 * p - 1) / (64 * p) > L1_MAX) p *= 2;
         return p;
     }
     static bool pl_valid(int p, int nv) {
-        return (p == 1 || p == 2 || p == 4 || p == 8 || p == 16 || p == 32) && (nv + 64 * p - 1) / (64 * p) <= 3072;
+        return (p == 1 || p == 2 || p == 4 || p == 8 || p == 16 || p == 32) && (nv + 64 * p - 1) / (64 * p) <= L1_MAX;
     }
     // Level sizes/offsets for width p, tree storage and the event loop's LDS.
     void layout_tree(int p) {
         if (!pl_valid(p, n))
-            throw HipError("too many voxels for the event tree (max 6.2M)", hipErrorUnknown, __FILE__, __LINE__);
+            throw HipError("too many voxels for the event tree (max 8.3M)", hipErrorUnknown, __FILE__, __LINE__);
         pl = p;
         lev_n[0] = n;
         lev_n[1] = (n + 64 * pl - 1) / (64 * pl);
