@@ -33,12 +33,6 @@ namespace ssa {
 
 constexpr int MAXS = 16;
 constexpr int MAXR = 16;
-// Diffusion events on rows of at most this many neighbours fetch every
-// neighbour's state while the destination walk runs; longer rows walk first
-// and load the destination alone (a 64k-voxel static lattice with 30
-// neighbours per voxel: 5.9 -> 4.9 us per event; 6-neighbour rows keep the
-// prefetch).
-constexpr int PREFETCH_ROW = 16;
 
 // ---------------------------------------------------------------------------
 // Random numbers.
@@ -808,8 +802,6 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
 
     // ---- LDS image
     Lds L;
-    uint32_t *nbx;   // prefetched neighbour counts [64][MS]
-    double *nbr;     // prefetched neighbour reaction rates [64][MR]
     uint32_t *vcode = nullptr, *vmx = nullptr;   // VM programs, staged counts
     double *vk = nullptr;                        // VM constants
     {
@@ -824,8 +816,6 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         L.prN = (int *)(smem + off); off = al16(off + sizeof(int) * nnzN);
         L.jcG = (int *)(smem + off); off = al16(off + sizeof(int) * (S + R + 1));
         L.irG = (int *)(smem + off); off = al16(off + sizeof(int) * nnzG);
-        nbx = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * WAVE * MS);
-        nbr = (double *)(smem + off); off = al16(off + sizeof(double) * WAVE * MR);
         if (VM && a.vm_ncode > 0) {
             vcode = (uint32_t *)(smem + off); off = al16(off + sizeof(uint32_t) * a.vm_ncode);
             vk = (double *)(smem + off); off = al16(off + sizeof(double) * a.vm_nk);
@@ -1087,32 +1077,12 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
             double rand2 = r2 * rl_d(Dd, spec);
             int kk = -1;             // chosen entry of the neighbour row
-            bool pref = false;       // its state is in the prefetch buffers
-            double n_sr = 0.0, n_sd = 0.0, n_Dd = 0.0, n_qt = inf;
-            int n_ty = 1;
             if (dg <= WAVE) {
-                // ---- round trip 2.  Short rows: every neighbour's state
-                // (lane k fetches neighbour k into LDS) issued before the walk
-                // picks one.  Longer rows: the walk runs first on the row held
-                // in registers and only the destination is loaded after, since
-                // a fetch of every neighbour costs more than the walk it hides.
-                const bool pf_all = dg <= PREFETCH_ROW;
+                // destination: walk the neighbour list (held in registers) in
+                // order; its state is loaded after (round trip 2).  Fetching
+                // every neighbour's state while the walk runs measured slower,
+                // for rows of 6 neighbours and of 30.
                 const bool have = lane < dg && nb_j >= 0;
-                if (have && pf_all) {
-                    const size_t j = (size_t)nb_j;
-#pragma unroll
-                    for (int s2 = 0; s2 < MS; s2++)
-                        if (s2 < S) nbx[lane * MS + s2] = xx_[j * S + s2];
-#pragma unroll
-                    for (int r = 0; r < MR; r++)
-                        if (r < R) nbr[lane * MR + r] = rrate_[j * R + r];
-                    n_sr = srrate_[j];
-                    n_sd = sdrate_[j];
-                    n_Dd = a.Ddiag[j * S + spec];
-                    n_qt = qtime_[j];
-                    n_ty = a.type[j];
-                }
-                // destination: walk the neighbour list in order
                 const double c = have ? nb_D * L.Dmat[spec * T + (nb_t - 1)] : 0.0;
                 for (int pass = 0; pass < 2 && kk < 0; pass++) {
                     double cum2 = 0.0;
@@ -1125,10 +1095,9 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
                         rand2 = r2 * cum2;
                     }
                 }
-                pref = pf_all && kk >= 0;
             } else {
                 // rows longer than a wavefront: the same in-order walk, 64
-                // entries at a time; the destination's state is loaded after
+                // entries at a time
                 for (int pass = 0; pass < 2 && kk < 0; pass++) {
                     double cum2 = 0.0;
                     for (int c0 = 0; c0 < dg && kk < 0; c0 += WAVE) {
@@ -1153,22 +1122,13 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             }
             if (kk < 0 && dest < 0) { err = SSASDPD_ERR_DIFFUSION_OVERFLOW; break; }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            // destination state (lane s / lane j layout): from the prefetch,
-            // or loaded (rows past the prefetch length, or a stale
-            // destination not in this row)
+            // destination state (lane s / lane j layout): the chosen entry, or
+            // the stale destination of an earlier event when the walk chose
+            // none
             uint32_t xd;
             double rrd, srd, sdd, Ddd, qtd;
             int tds;
-            if (pref) {
-                dest = __builtin_amdgcn_readlane(nb_j, kk);
-                xd = s_lane ? nbx[kk * MS + lane] : 0u;
-                rrd = r_lane ? nbr[kk * MR + lane] : 0.0;
-                srd = rl_d(n_sr, kk);
-                sdd = rl_d(n_sd, kk);
-                Ddd = rl_d(n_Dd, kk);
-                qtd = rl_d(n_qt, kk);
-                tds = __builtin_amdgcn_readlane(n_ty, kk);
-            } else {
+            {
                 if (kk >= 0)
                     dest = dg <= WAVE ? __builtin_amdgcn_readlane(nb_j, kk)
                                       : __builtin_amdgcn_readfirstlane(a.erow[(size_t)sv * W + kk].j);
@@ -1381,8 +1341,6 @@ size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG, in
     off = a16(off + sizeof(int) * nnzN);
     off = a16(off + sizeof(int) * (S + R + 1));
     off = a16(off + sizeof(int) * nnzG);
-    off = a16(off + sizeof(uint32_t) * 64 * 16);
-    off = a16(off + sizeof(double) * 64 * 16);
     if (n_code > 0) {
         off = a16(off + sizeof(uint32_t) * n_code);
         off = a16(off + sizeof(double) * n_consts);
