@@ -307,6 +307,10 @@ struct ssasdpd_engine {
             tree_nodes += lev_n[nlev + 1];
             nlev++;
         }
+        // the event loop holds level nlev-1 in registers and repairs the
+        // levels below it with at most one level in LDS
+        if (nlev > 3)
+            throw HipError("event tree deeper than three levels", hipErrorUnknown, __FILE__, __LINE__);
         dfree(tmin); dfree(targ);
         tmin = dalloc<double>(tree_nodes);
         targ = dalloc<int>(tree_nodes);

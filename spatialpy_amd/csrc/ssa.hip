@@ -714,11 +714,9 @@ __device__ __forceinline__ void level1_get(const TreeShape &a, const Lds &L, con
 }
 
 // Voxel v gets time tv; its level-0 node is in sib (already holding tv).
-// known: v is the voxel the event loop selected, whose level-1 node's
-// minimum was (old_t, v) -- the root -- so the stored value is not read.
 template <int PL>
 __device__ void tree_update_rescan(const TreeShape &a, double *qtime, Lds &L, TreeTop &tp, const Sib<PL> &sib, int v,
-                                   double tv, int lane, bool known = false, double old_t = 0.0) {
+                                   double tv, int lane) {
     if (lane == 0) qtime[v] = tv;
     double m;
     int am;
@@ -727,10 +725,72 @@ __device__ void tree_update_rescan(const TreeShape &a, double *qtime, Lds &L, Tr
     const int off = a.off(1) + idx;
     double om;
     int oam;
-    if (known) { om = old_t; oam = v; }
-    else level1_get(a, L, tp, idx, om, oam);
+    level1_get(a, L, tp, idx, om, oam);
     if (lane == 0) { L.lt[off] = m; L.lv[off] = am; }
     if (!(m == om && am == oam)) tree_fix_up<PL>(a, L, tp, idx, m, am, om, oam, lane);
+}
+
+// The selected voxel v is the global minimum, so every ancestor's minimum
+// comes from it, and its new time tv rescans all of them.  Each ancestor's
+// new minimum is min((v's new node minimum), (the ancestor's minimum over
+// its other children)), and those other-children minima read only LDS and
+// registers: pop_prepare computes them while the event's first memory round
+// trip is in flight, pop_finish merges them once the level-0 node is in.
+// (time, voxel) pairs merge lexicographically, which is the lowest-lane rule
+// of the wave reductions, since lanes hold increasing voxel ranges.  Level 1
+// holds at most 4096 nodes, so the register level is level 1 or 2.
+struct PopPre {
+    double t1;   // the level-1 node's other children (register level 2 only)
+    int v1;
+    double t2;   // the register level's other nodes
+    int v2;
+};
+
+template <int PL>
+__device__ __forceinline__ PopPre pop_prepare(const TreeShape &a, const Lds &L, const TreeTop &tp, int v, int lane) {
+    PopPre r;
+    const int idx = v / (WAVE * PL);
+    int top = idx;   // v's ancestor at the register level
+    r.t1 = INFINITY;
+    r.v1 = INT_MAX;
+    if (tp.regl != 1) {
+        top = idx / WAVE;
+        const int c = top * WAVE + lane;
+        double t = INFINITY;
+        int vv = INT_MAX;
+        if (c < a.cnt(1) && c != idx) { t = L.lt[a.off(1) + c]; vv = L.lv[a.off(1) + c]; }
+        wave_argmin_dpp(t, vv);
+        r.t1 = t;
+        r.v1 = vv;
+    }
+    double t = lane == top ? INFINITY : tp.rt;
+    int vv = lane == top ? INT_MAX : tp.rv;
+    wave_argmin_dpp(t, vv);
+    r.t2 = t;
+    r.v2 = vv;
+    return r;
+}
+
+template <int PL>
+__device__ __forceinline__ void pop_finish(const TreeShape &a, double *qtime, Lds &L, TreeTop &tp, const Sib<PL> &sib,
+                                           int v, double tv, const PopPre &pre, int lane) {
+    if (lane == 0) qtime[v] = tv;
+    double m;
+    int am;
+    sib.argmin(m, am, lane, a.n);
+    const int idx = v / (WAVE * PL);
+    if (lane == 0) { L.lt[a.off(1) + idx] = m; L.lv[a.off(1) + idx] = am; }
+    int top = idx;
+    if (tp.regl != 1) {
+        argmin2(m, am, pre.t1, pre.v1);   // the level-1 node's new minimum
+        top = idx / WAVE;
+        if (lane == 0) { L.lt[a.off(2) + top] = m; L.lv[a.off(2) + top] = am; }
+    }
+    if (lane == top) { tp.rt = m; tp.rv = am; }
+    argmin2(m, am, pre.t2, pre.v2);       // the root
+    tp.root_t = m;
+    tp.root_v = am;
+    if (lane == 0) { L.lt[tp.root] = m; L.lv[tp.root] = am; }
 }
 
 // Voxel v gets time tv without its level-0 node in registers: the node's
@@ -964,6 +1024,9 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         const int nbj_raw = er.j;
         const double nbD_raw = er.D;
         const int nbt_raw = er.type;
+        // while the loads above are in flight: the selected voxel's ancestors
+        // without it
+        const PopPre pre = pop_prepare<PL>(tsh, L, tp, sv, lane);
         uint32_t xs = s_lane ? xs_raw : 0u;
         const double Dd = s_lane ? Dd_raw : 0.0;
         double rr = r_lane ? rr_raw : 0.0;
@@ -1042,7 +1105,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(tsh, qtime_, L, tp, sib, sv, ntv, lane, true, tt);
+            pop_finish<PL>(tsh, qtime_, L, tp, sib, sv, ntv, pre, lane);
         } else {
             // ---- diffusion (simulate_rdme.cpp:312-445)
             event = 1;
@@ -1211,7 +1274,7 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
             if (tot <= 0.0) { ntv = inf; q.active--; }   // the selected voxel's time was finite
             else ntv = rng.exponential(lane) / tot + tt;
             sib.set(sv, ntv, lane);
-            tree_update_rescan<PL>(tsh, qtime_, L, tp, sib, sv, ntv, lane, true, tt);
+            pop_finish<PL>(tsh, qtime_, L, tp, sib, sv, ntv, pre, lane);
             const double dest_tot = srd + sdd;
             const double dold = self ? ntv : qtd;
             double ntd;
