@@ -1025,8 +1025,10 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         const double nbD_raw = er.D;
         const int nbt_raw = er.type;
         // while the loads above are in flight: the selected voxel's ancestors
-        // without it
+        // without it, and the event's first random draw
         const PopPre pre = pop_prepare<PL>(tsh, L, tp, sv, lane);
+        const double rand1 = rng.uniform(lane);   // the event's first draw
+        if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
         uint32_t xs = s_lane ? xs_raw : 0u;
         const double Dd = s_lane ? Dd_raw : 0.0;
         double rr = r_lane ? rr_raw : 0.0;
@@ -1034,8 +1036,6 @@ __device__ __forceinline__ void ssa_run(const RdmeArgs &a, const size_t b) {
         const double nb_D = lane < RW ? nbD_raw : 0.0;
         const int nb_t = (lane < RW && T > 1) ? nbt_raw : 1;
         double totrate = sr + sd;
-        const double rand1 = rng.uniform(lane);
-        if (rng.exhausted) { err = SSASDPD_ERR_REPLAY_EXHAUSTED; break; }
         int event;
         int which;
         if (rand1 <= sr / totrate) {
