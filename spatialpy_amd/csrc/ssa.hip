@@ -520,13 +520,14 @@ template <int MR>
 __device__ __forceinline__ double propensity_par(const LaneModel<MR> &m, uint32_t x_l, double vol, int tp) {
     const int xa = __shfl((int)x_l, m.ra, WAVE);
     const int xb = __shfl((int)x_l, m.rb, WAVE);
-    if (m.mask && !((m.mask >> tp) & 1ULL)) return 0.0;
-    switch (m.kind) {
-    case 0: return m.k * vol;
-    case 1: return m.k * xa;
-    case 2: return ((m.k * xa) * xb) / vol;
-    default: return ((m.k * xa) * (xa - 1)) / vol;
-    }
+    // every kind's operations on every lane, then a select: lanes of
+    // different kinds share one division instead of diverging
+    const double t1 = m.k * xa;                                  // kinds 1-3
+    const double t2 = t1 * (m.kind == 2 ? xb : xa - 1);          // kinds 2, 3
+    const double t3 = t2 / vol;
+    double p = m.kind == 0 ? m.k * vol : (m.kind == 1 ? t1 : t3);
+    if (m.mask && !((m.mask >> tp) & 1ULL)) p = 0.0;
+    return p;
 }
 // propensity_par with kind-4 rows (custom propensities) from the expression
 // VM, for the lanes in `want`: the counts are staged in LDS (vmx) so that
