@@ -123,3 +123,34 @@ def test_moving_dam_break_small_matches_oracle(oracle):
     assert np.allclose(res["v"], ref["v"], rtol=1e-9, atol=1e-12)
     assert np.allclose(res["rho"], ref["rho"], rtol=1e-12, atol=0)
     assert np.array_equal(res["xx"], ref["xx"])
+
+
+@pytest.mark.parametrize("case,width", [("static18", 0), ("cylinder20k", 0), ("cylinder20k", 4)])
+def test_large_event_tree_replay_matches_oracle(case, width, oracle, monkeypatch):
+    """Domains with more than 64 level-1 nodes in the event tree, so the
+    level held in registers is level 2 (as on the 1M cylinder and the 64k
+    lattice): event-by-event parity with the oracle on a static 18^3
+    lattice (5,832 voxels) and a moving 20k-particle cylinder (ANN tie
+    order), the latter also at event-tree width 4."""
+    if width:
+        monkeypatch.setenv("SSASDPD_EVENT_TREE_PL", str(width))
+    if case == "static18":
+        sa = workloads.static_rd_3d(n_side=18, nA=1500, nB=1500, t_end=1, num_points=2, seed=4)
+        o = oracle.Oracle(sa)
+    else:
+        sa = workloads.cylinder_3d(n_target=20_000, nt=6, seed=3)
+        sa.flags = 2   # ANN tie order on the moving domain, as the oracle
+        o = oracle.Oracle(sa, gauss_seidel=False)
+    n_l1 = -(-sa.n // (64 * (width or 1)))
+    assert n_l1 > 64
+    words, expo = mt19937_64_stream(13, 2_000_000)
+    eng = Engine(sa, trace_capacity=400_000)
+    res = eng.run(words=words, expo=expo)
+    ev, nev = eng.trace(400_000)
+    o.set_trace(400_000)
+    ref = o.run(seed=13)
+    oev, onev = o.trace(400_000)
+    assert nev == onev and nev > 200, (nev, onev)
+    assert np.array_equal(ev, oev)
+    assert np.array_equal(res["xx"], ref["xx"])
+    assert eng.stats().rng_words_used == o.stats().rng_words_used
