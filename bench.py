@@ -10,10 +10,11 @@ Prints ONE JSON line (metric particle-steps/s, higher is better), with the
 SSA event rate, the roofline of the dominant HBM-bound kernels and the
 reference CPU engine timed on this host's cores beside it.
 
-Two more workloads print the same kind of line: ``--workload static3d``
+Three more workloads print the same kind of line: ``--workload static3d``
 (the 40^3 static A+B<->C case: SSA events/s of a seed batch, with one
-trajectory beside it) and ``--workload ensemble`` (uneven gene-network
-models in one persistent launch: trajectories/s).
+trajectory beside it), ``--workload ensemble`` (uneven gene-network models
+in one persistent launch: trajectories/s) and ``--workload dambreak`` (the
+2D SDPD dam break with walls: particle-steps/s, FP32 pair forces beside).
 """
 import argparse
 import json
@@ -315,24 +316,110 @@ def bench_ensemble(args):
     print(json.dumps(out))
 
 
+# The dam break's time step: at the configured 1e-4 the reference engine and
+# the oracle both turn non-finite by step 12 (above the acoustic limit
+# 0.25 h / c0 = 8e-5), and at 5e-5 the density triples within 200 steps;
+# at 2.5e-5 it stays within 10% of rho0.
+DAMBREAK_DT = 2.5e-5
+
+
+def bench_dambreak(args):
+    """The 2D SDPD dam break with transport: 200x200 fluid particles at
+    spacing 0.0025 inside a 2-particle-thick fixed wall around [0,2]x[0,1.5],
+    h = 1.3 spacing, dt = DAMBREAK_DT, A->0 on 1000 molecules.  FP64 pair
+    forces (the reference's arithmetic) for `value`, the FP32 pair-force
+    option timed beside it."""
+    from spatialpy_amd import workloads, _abi
+    from spatialpy_amd.engine import Engine
+    steps = args.steps if args.steps is not None else 400
+    warm = args.warmup if args.warmup is not None else 20
+    total = warm + steps
+
+    def timed(fp32):
+        sa = workloads.dam_break_2d(nt=total + 1, out_every=total + 1, dt=DAMBREAK_DT)
+        if fp32:
+            sa.flags |= _abi.FLAG_FP32_FORCES
+        eng = Engine(sa)
+        eng.reset(seed=12345)
+        eng.advance(warm)
+        ph0, st0 = eng.phase_times(), eng.stats()
+        t0 = time.perf_counter()
+        eng.advance(steps)
+        sec = time.perf_counter() - t0
+        ph1, st1 = eng.phase_times(), eng.stats()
+        eng.close()
+        return sa, sec, {k: ph1[k][0] - ph0[k][0] for k in ph1}, (st1.reactions - st0.reactions,
+                                                                  st1.diffusions - st0.diffusions)
+
+    sa, sec, ph, (n_re, n_di) = timed(False)
+    _, sec32, ph32, _ = timed(True)
+    n = sa.n
+    E = mean_degree(sa.x, sa.h)
+    B = algorithmic_bytes_per_particle_step(2, sa.n_species, E)
+    hbm_ms = ph["search"] + ph["sdpd"] + ph["forces"] + ph["rates_queue"] + ph["bin_tree"]
+    achieved = B * n * steps / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
+    particle_roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernels": "binning + fixed-radius search + D_ij + SDPD substeps + rates/queue build",
+                     "algorithmic_bytes_per_particle_step": round(B, 1), "mean_degree": round(E, 2),
+                     "ms_per_step": round(hbm_ms / steps, 4)}
+    ssa_roof = ssa_roofline(sa.n_species, sa.n_reactions, E, n_re, n_di, ph["ssa"],
+                            "k_ssa (event loop; latency-bound, one wavefront per trajectory)")
+    ssa_roof["ms_per_step"] = round(ph["ssa"] / steps, 4)
+    out = {
+        "metric": "particle-steps/s", "value": n * steps / sec, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": 1e3 * sec / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D SDPD dam break with transport: 200x200 fluid particles at spacing 0.0025 and a "
+                               f"2-particle-thick fixed wall around [0,2]x[0,1.5] ({n} particles), h = 1.3 spacing, "
+                               f"rho0 = 1, c0 = 10, nu = 1e-3, gravity -1, dt = {DAMBREAK_DT} (1e-4 turns non-finite by "
+                               f"step 12 in the reference engine too), A->0 k = 0.05 on 1000 molecules; "
+                               f"FP64 pair forces", "particles": n, "dt": DAMBREAK_DT},
+        "phase_ms_per_step": {k: round(v / steps, 4) for k, v in ph.items()},
+        "fp32_forces": {"value": n * steps / sec32, "ms_per_step": 1e3 * sec32 / steps,
+                        "forces_ms_per_step": round(ph32["forces"] / steps, 4),
+                        "note": "FP32 pair terms from FP64-formed dx and dv, FP64 sums (within 1e-5 of the FP64 "
+                                "oracle in tests/test_gpu_forces_fp32.py)"},
+        "ssa_events_per_s": (n_re + n_di) / sec,
+        "dominant": "k_ssa" if ph["ssa"] > hbm_ms else "particle_path",
+        "roofline": ssa_roof if ph["ssa"] > hbm_ms else particle_roof,
+        "roofline_particle_path": particle_roof,
+        "roofline_ssa": ssa_roof,
+    }
+    if not args.no_cpu_baseline:
+        k = args.cpu_steps if args.cpu_steps else 20
+        csa = workloads.dam_break_2d(nt=k + 1, out_every=k + 1, dt=DAMBREAK_DT)
+        r = reference_timing(csa, k)
+        kind = "reference"
+        if r is None:
+            r, kind = oracle_timing(csa, k), "port"
+        out["cpu_baseline"] = {"value": n * k / r[0], "unit": "particle-steps/s", "cores": r[2], "kind": kind,
+                               "sample": f"the same dam break, {k} steps from t = 0: {r[0]:.2f} s"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("cylinder", "static3d", "ensemble"), default="cylinder",
+    ap.add_argument("--workload", choices=("cylinder", "static3d", "ensemble", "dambreak"), default="cylinder",
                     help="cylinder: the 1M-particle moving SDPD+sSSA run (the headline line); static3d: the "
                          "64k-voxel static A+B<->C case, one trajectory and a seed batch; ensemble: uneven "
-                         "gene-network models in one launch")
+                         "gene-network models in one launch; dambreak: the 2D SDPD dam break with walls")
     # the open cylinder at h = 1.3 x spacing, c0 = 20, dt = 2.5e-4 turns
     # non-finite at step 21 in the reference engine itself (and identically
     # here), so the default window stays inside steps 0..17
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (cylinder 15, static3d 2)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (cylinder 2, static3d 0)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (cylinder 15, static3d 2, dambreak 400)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (cylinder 2, static3d 0, "
+                                                              "dambreak 20)")
     ap.add_argument("--n", type=int, default=1_000_000, help="particles in the cylinder")
     ap.add_argument("--batch", type=int, default=256, help="static3d: trajectories in the seed batch")
     ap.add_argument("--models", type=int, default=256, help="ensemble: number of models")
     ap.add_argument("--t-end", type=int, default=10, help="ensemble: simulated time (unit steps)")
     ap.add_argument("--cpu-models", type=int, default=8, help="ensemble: models in the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline sample (cylinder 2, "
+                                                                   "dambreak 20)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate forward-pass timing")
@@ -360,8 +447,11 @@ def main():
         return bench_static3d(args)
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "dambreak":
+        return bench_dambreak(args)
     args.steps = 15 if args.steps is None else args.steps
     args.warmup = 2 if args.warmup is None else args.warmup
+    args.cpu_steps = 2 if args.cpu_steps is None else args.cpu_steps
     if args.warmup + args.steps > 20:
         print("warning: this workload turns non-finite at step 21 (also in the reference engine); "
               "keep warmup + steps <= 20", file=sys.stderr)
