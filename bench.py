@@ -264,7 +264,11 @@ def bench_ensemble(args):
     rng = np.random.default_rng(0)
     sizes = rng.integers(8000, 64001, size=M)
     t0 = time.perf_counter()
-    models = [workloads.gene_network_3d(n_target=int(nv), seed=1000 + i, t_end=t_end) for i, nv in enumerate(sizes)]
+    models = []
+    for i, nv in enumerate(sizes):
+        models.append(workloads.gene_network_3d(n_target=int(nv), seed=1000 + i, t_end=t_end))
+        if (i + 1) % 128 == 0:
+            print(f"built {i + 1} of {M} models", file=sys.stderr, flush=True)
     build_s = time.perf_counter() - t0
     print(f"built {M} models in {build_s:.1f} s", file=sys.stderr, flush=True)
     E = float(np.mean([mean_degree(m.x, m.h, samples=500, seed=i) for i, m in enumerate(models[:16])]))
