@@ -419,7 +419,7 @@ def main():
                                                               "dambreak 20)")
     ap.add_argument("--n", type=int, default=1_000_000, help="particles in the cylinder")
     ap.add_argument("--batch", type=int, default=256, help="static3d: trajectories in the seed batch")
-    ap.add_argument("--models", type=int, default=256, help="ensemble: number of models")
+    ap.add_argument("--models", type=int, default=1024, help="ensemble: number of models")
     ap.add_argument("--t-end", type=int, default=10, help="ensemble: simulated time (unit steps)")
     ap.add_argument("--cpu-models", type=int, default=8, help="ensemble: models in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline sample (cylinder 2, "
