@@ -93,6 +93,15 @@ struct ssasdpd_engine {
     int *nidx = nullptr, *deg = nullptr;
     double *nr2 = nullptr, *nr = nullptr, *Dij = nullptr;
     EdgeRec *erow = nullptr;   // the rows voxel-major, for the event loop
+    // Look-ahead (advance on moving domains): the next step's particle phase
+    // runs on its own stream while this step's event loop runs.  The event
+    // loop reads only deg and erow of the lists, so those two are double
+    // buffered; it keeps the previous step's pair.
+    hipStream_t ps = nullptr;
+    hipEvent_t la_done = nullptr;
+    bool la_pending = false;
+    int *deg_alt = nullptr;
+    EdgeRec *erow_alt = nullptr;
     // binning / tree
     double *xtree = nullptr, *spts = nullptr, *d_bbox = nullptr;
     unsigned long long *d_bbtmp = nullptr;
@@ -184,7 +193,13 @@ struct ssasdpd_engine {
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
         dfree(rho); dfree(old_rho); dfree(Frho); dfree(mass); dfree(nu); dfree(bvf);
         dfree(type); dfree(solid); dfree(C); dfree(Q);
+        if (ps) (void)hipStreamSynchronize(ps);
         dfree(nidx); dfree(deg); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(erow); dfree(vol);
+        dfree(deg_alt); dfree(erow_alt);
+        if (la_done) (void)hipEventDestroy(la_done);
+        la_done = nullptr;
+        if (ps) (void)hipStreamDestroy(ps);
+        ps = nullptr;
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
         dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
         dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
@@ -207,7 +222,9 @@ struct ssasdpd_engine {
     }
 
     void alloc_lists(int w) {
-        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(erow);
+        // the event loop of a step may still read the other row buffer
+        if (ps) HIP_OK(hipDeviceSynchronize());
+        dfree(nidx); dfree(nr2); dfree(nr); dfree(Dij); dfree(ntype); dfree(erow); dfree(erow_alt);
         W = w;
         size_t cap = (size_t)W * n;
         nidx = dalloc<int>(cap);
@@ -216,6 +233,7 @@ struct ssasdpd_engine {
         nr = dalloc<double>(cap);
         Dij = dalloc<double>(cap);
         erow = dalloc<EdgeRec>(cap);
+        if (ps) erow_alt = dalloc<EdgeRec>(cap);
     }
 
     // options set before setup (ensembles): a common event-tree width and a
@@ -335,7 +353,11 @@ struct ssasdpd_engine {
         alpha = kernel_alpha(dim, h);
         out_steps.assign(d->output_steps, d->output_steps + d->n_output_steps);
         if (ext_stream) st = ext_stream;
-        else HIP_OK(hipStreamCreate(&st));
+        else {
+            HIP_OK(hipStreamCreate(&st));
+            HIP_OK(hipStreamCreate(&ps));
+            HIP_OK(hipEventCreateWithFlags(&la_done, hipEventDisableTiming));
+        }
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++) {
                 HIP_OK(hipEventCreate(&ev_a[p][k]));
@@ -349,6 +371,7 @@ struct ssasdpd_engine {
         type = dalloc<int>(n); solid = dalloc<int>(n);
         C = dalloc<double>((size_t)n * Sc); Q = dalloc<double>((size_t)n * Sc);
         deg = dalloc<int>(n);
+        if (ps) deg_alt = dalloc<int>(n);
         sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
         d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
         derr = dalloc<DevErr>(1);
@@ -450,6 +473,8 @@ struct ssasdpd_engine {
     }
 
     void reset(const ssasdpd_rng *r) {
+        if (ps) HIP_OK(hipStreamSynchronize(ps));
+        la_pending = false;
         size_t n3 = 3 * (size_t)n;
         h2d(st, x, x0.data(), n3);
         h2d(st, v, v0.data(), n3);
@@ -599,8 +624,11 @@ struct ssasdpd_engine {
         return a;
     }
 
-    int check(unsigned step) {
-        DevErr e = read_err();
+    // particles: also report the particle phase's errors (off when a
+    // look-ahead has already run the next step's particle phase).
+    int check(unsigned step, bool particles = true) {
+        DevErr e{};
+        if (particles) e = read_err();
         if (e.code) {
             stats.bad_particle = e.particle == INT32_MAX ? -1 : e.particle;
             char buf[160];
@@ -767,9 +795,17 @@ struct ssasdpd_engine {
         return rdme_initialized;
     }
 
-    int do_step() {
+    // One step.  lookahead: the caller runs the next step right after, so
+    // its particle phase may start now, beside this step's event loop.
+    int do_step(bool lookahead = false) {
         const unsigned step = cur;
-        particle_phase(step);
+        if (la_pending) {
+            HIP_OK(hipStreamWaitEvent(st, la_done, 0));
+            la_pending = false;
+        } else {
+            particle_phase(step);
+        }
+        bool la = false;
         if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
             RdmeArgs ra = rdme_args(step);
             if (static_domain == 0 || !rdme_initialized) {
@@ -779,14 +815,28 @@ struct ssasdpd_engine {
                 ph_end(3);
                 QueueState q;
                 d2h(st, &q, qs, 1);
+                // this step's particle errors, read before a look-ahead can add the next step's
+                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt;
+                DevErr pe{};
+                if (want) d2h(st, &pe, derr, 1);
                 HIP_OK(hipStreamSynchronize(st));
                 if (q.initialized == 1) rdme_initialized = true;
+                la = want && pe.code == 0;
             }
             ph_begin(4);
             rdme_take_step(st, ra);
             ph_end(4);
         }
-        int rc = check(step);
+        if (la) {
+            std::swap(deg, deg_alt);
+            std::swap(erow, erow_alt);
+            std::swap(st, ps);
+            particle_phase(step + 1);
+            HIP_OK(hipEventRecord(la_done, st));
+            std::swap(st, ps);
+            la_pending = true;
+        }
+        int rc = check(step, !la);
         end_step();
         return rc;
     }
@@ -1198,7 +1248,7 @@ int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps) {
             if (rc) return rc;
             break;
         }
-        int rc = e->do_step();
+        int rc = e->do_step(e->cur + 1 < target);
         if (rc) return rc;
     }
     e->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -154,3 +154,26 @@ def test_large_event_tree_replay_matches_oracle(case, width, oracle, monkeypatch
     assert np.array_equal(ev, oev)
     assert np.array_equal(res["xx"], ref["xx"])
     assert eng.stats().rng_words_used == o.stats().rng_words_used
+
+
+@pytest.mark.parametrize("case", ["dam_break", "cylinder20k"])
+def test_advance_with_particle_lookahead_equals_run(case):
+    """advance() starts the next step's particle phase on a second stream
+    while the current step's event loop runs (moving domains); its state
+    must equal run()'s, which takes the steps one after another, bit for
+    bit.  Two advance() calls also cover the step where a call ends (no
+    look-ahead there)."""
+    if case == "dam_break":
+        sa = workloads.dam_break_2d(nx=20, ny=20, spacing=0.025, n_mol=100, nt=45, out_every=45, dt=2e-4)
+    else:
+        sa = workloads.cylinder_3d(n_target=20_000, nt=6, seed=3)
+    words, expo = mt19937_64_stream(17, 600_000)
+    ref = Engine(sa).run(words=words, expo=expo, fields=("xx", "x", "v", "rho"))
+    eng = Engine(sa)
+    eng.reset(words=words, expo=expo)
+    eng.advance(sa.nt // 2)
+    eng.advance(sa.nt - sa.nt // 2)
+    st = eng.state()
+    assert np.array_equal(st["xx"], ref["xx"][-1])
+    for f in ("x", "v", "rho"):
+        assert np.array_equal(st[f].view(np.uint64), ref[f][-1].view(np.uint64)), f
