@@ -104,6 +104,10 @@ struct ssasdpd_engine {
     EdgeRec *erow_alt = nullptr;
     // binning / tree
     double *xtree = nullptr, *spts = nullptr, *d_bbox = nullptr;
+    // pinned landing slot for the per-step bbox readback: a pageable copy goes
+    // through the runtime's shared staging buffer, which can hold the particle
+    // stream behind a running event loop
+    double *h_bbox = nullptr;
     unsigned long long *d_bbtmp = nullptr;
     int *sorted = nullptr, *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
     long long cell_cap = 0;
@@ -201,6 +205,8 @@ struct ssasdpd_engine {
         if (ps) (void)hipStreamDestroy(ps);
         ps = nullptr;
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
+        if (h_bbox) (void)hipHostFree(h_bbox);
+        h_bbox = nullptr;
         dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
         dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
         if (kd_work) (void)hipFree(kd_work);
@@ -374,6 +380,7 @@ struct ssasdpd_engine {
         if (ps) deg_alt = dalloc<int>(n);
         sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
         d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
+        HIP_OK(hipHostMalloc((void **)&h_bbox, 8 * sizeof(double), hipHostMallocDefault));
         derr = dalloc<DevErr>(1);
         d_psum = dalloc<double>(2);
         kd.parent = dalloc<int>(2 * (size_t)n + 2); kd.depth = dalloc<int>(2 * (size_t)n + 2);
@@ -543,13 +550,15 @@ struct ssasdpd_engine {
         ph_begin(0);
         HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
         compute_bbox(st, xtree, n, dim, d_bbox, d_bbtmp);
-        double bb[6];
-        d2h(st, bb, d_bbox, 6);
+        d2h(st, h_bbox, d_bbox, 6);
         HIP_OK(hipStreamSynchronize(st));
-        grid_setup(bb, n, dim, h, grid);
+        grid_setup(h_bbox, n, dim, h, grid);
         if (grid.ncell + 1 > cell_cap) {
+            // hipFree waits for the whole device (a running event loop too), so
+            // grow with a quarter of headroom: a slowly widening bounding box
+            // then reallocates rarely instead of every few steps
             dfree(cell_count); dfree(cell_start); dfree(cell_fill);
-            cell_cap = grid.ncell + 1;
+            cell_cap = grid.ncell + 1 + grid.ncell / 4 + 64;
             cell_count = dalloc<int>(cell_cap);
             cell_start = dalloc<int>(cell_cap + 1);
             cell_fill = dalloc<int>(cell_cap);
