@@ -1,0 +1,85 @@
+"""The 1M-particle cylinder (A+B->C in a 3D SDPD pipe inflow, 1,005,649
+particles) at its full size, through properties that hold at any size:
+
+* fixed-radius search (particle.cpp:240-294): the neighbour set of every
+  particle equals scipy's cKDTree pair set within h (an independent
+  library routine), and each r^2 equals dx*dx + dy*dy + dz*dz summed left
+  to right, bit for bit;
+* A+B->C (one reaction, no inflow of molecules): at every step the totals
+  obey A = A0 - R, B = B0 - R and C = R, with R the engine's reaction count;
+* the seeded (counter-based) stream makes a run repeatable: two engines on
+  the same seed end in the same state, bit for bit, whether the steps are
+  taken in one advance() call or two."""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from spatialpy_amd import workloads
+from spatialpy_amd.engine import Engine, fr_search
+
+pytestmark = pytest.mark.gpu
+
+N_FULL = 1_005_649
+
+
+@pytest.fixture(scope="module")
+def cylinder():
+    sa = workloads.cylinder_3d(n_target=1_000_000, nt=4)
+    assert sa.n == N_FULL
+    return sa
+
+
+def test_full_size_search_matches_kdtree(cylinder):
+    sa = cylinder
+    rng = np.random.default_rng(11)
+    # positions off the lattice, so that no pair sits exactly at h
+    pts = sa.x + rng.normal(0.0, 0.05 * sa.h, size=sa.x.shape)
+    rp, idx, r2 = fr_search(pts, pts, 3, sa.h)
+    deg = np.diff(rp)
+    assert deg.min() >= 1 and rp[-1] > 5 * sa.n
+    rows = np.repeat(np.arange(sa.n, dtype=np.int64), deg)
+
+    pairs = cKDTree(pts).query_pairs(sa.h, output_type="ndarray").astype(np.int64)
+    ri = np.concatenate([pairs[:, 0], pairs[:, 1]])
+    rj = np.concatenate([pairs[:, 1], pairs[:, 0]])
+    o1 = np.lexsort((idx, rows))
+    o2 = np.lexsort((rj, ri))
+    assert np.array_equal(rows[o1], ri[o2])
+    assert np.array_equal(idx[o1].astype(np.int64), rj[o2])
+
+    i, j = rows[o1], idx[o1]
+    d = pts[i] - pts[j]
+    want = d[:, 0] * d[:, 0]
+    want = want + d[:, 1] * d[:, 1]
+    want = want + d[:, 2] * d[:, 2]
+    assert np.array_equal(r2[o1].view(np.uint64), want.view(np.uint64))
+
+
+def test_full_size_cylinder_conserves_molecules_and_repeats(cylinder):
+    sa = cylinder
+    A0, B0 = (int(v) for v in sa.u0[:, :2].astype(np.int64).sum(axis=0))
+    assert A0 > 0 and B0 > 0 and int(sa.u0[:, 2].sum()) == 0
+
+    def totals(eng):
+        t = eng.state()["xx"].astype(np.int64).sum(axis=0)
+        return int(t[0]), int(t[1]), int(t[2])
+
+    eng = Engine(sa)
+    eng.reset(seed=5)
+    assert totals(eng) == (A0, B0, 0)
+    for k in range(2):
+        eng.advance(2)
+        R = int(eng.stats().reactions)
+        assert R > 0
+        assert totals(eng) == (A0 - R, B0 - R, R), k
+    st = eng.state()
+    assert np.isfinite(st["x"]).all() and np.isfinite(st["v"]).all() and np.isfinite(st["rho"]).all()
+    assert eng.stats().diffusions > 0
+
+    again = Engine(sa)
+    again.reset(seed=5)
+    again.advance(4)
+    st2 = again.state()
+    assert np.array_equal(st["xx"], st2["xx"])
+    for f in ("x", "v", "rho"):
+        assert np.array_equal(st[f].view(np.uint64), st2[f].view(np.uint64)), f
