@@ -61,16 +61,35 @@ def build_workload(n_target, nt):
     return workloads.cylinder_3d(n_target=n_target, nt=nt)
 
 
-def cpu_baseline(sa, steps, seconds_budget=30.0):
-    """Time the reference engine itself (oracle/_ref, compiled from the
-    reference sources in this repository's build step) on this host's cores,
-    on a bounded sample: the same workload for `steps` steps."""
+# The cylinder run: CYL_RUN_STEPS steps in all (the configured 2,000-step
+# plain run).  The first CYL_SETTLE steps run untimed: the first ~100 are the
+# transient in which the A and B molecules that start in the same voxel
+# react (about 490k reactions in the first 50 steps, against a few thousand
+# per 50 steps from step 1000 on).  The timed window starts at step
+# CYL_SETTLE + warmup, in the middle of the run; the whole run's rate is
+# reported beside it.
+CYL_RUN_STEPS = 2000
+CYL_SETTLE = 1000
+
+
+def harness_exe():
+    return os.path.join(ROOT, "oracle", "_ref", "ref_harness_O2")
+
+
+def cpu_baseline(sa, state, start_step, steps):
+    """Time the reference engine itself (oracle/_ref/ref_harness_O2, the
+    reference's engine sources compiled by oracle/Makefile) on this host's
+    cores, over a bounded sample of the GPU's own timed window: `steps` steps
+    starting at `start_step` from the GPU's particle state and counts at
+    that step.  Without that build the repository's FP64 restatement (one
+    thread) is timed instead, and the line says so (kind "port")."""
     threads = min(os.cpu_count() or 1, 16)
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_O2")
+    exe = harness_exe()
     from spatialpy_amd import lowering
     if os.path.exists(exe):
         with tempfile.TemporaryDirectory() as d:
-            lowering.write_harness_dir(sa, d, seed=1, threads=threads, run_steps=steps)
+            lowering.write_harness_dir(sa, d, seed=1, threads=threads, run_steps=steps, state=state,
+                                       start_step=start_step)
             r = subprocess.run([exe, d], capture_output=True, text=True, timeout=600)
             line = [l for l in r.stdout.splitlines() if l.startswith("REF_TIMING")]
             if r.returncode == 0 and line:
@@ -80,13 +99,18 @@ def cpu_baseline(sa, steps, seconds_budget=30.0):
                 return {"value": sa.n * steps / sec, "unit": "particle-steps/s", "cores": threads,
                         "kind": "reference",
                         "sample": f"reference C++ engine (-O2, {threads} threads, run_simulation worker "
-                                  f"structure without VTK output) on the same {sa.n}-particle cylinder, "
-                                  f"{steps} steps from t=0: {sec:.2f} s",
+                                  f"structure without VTK output) on the same {sa.n}-particle cylinder, steps "
+                                  f"{start_step}..{start_step + steps - 1} (the first {steps} of the GPU's timed "
+                                  f"window) from the GPU's state at step {start_step}: {sec:.2f} s",
                         "ssa_events_per_s": ev / sec,
+                        "ssa_events": ev,
                         "ssa_events_per_s_in_rdme": ev / max(float(kv["rdme_s"]), 1e-9),
                         "kd_tree_s": float(kv["kd_s"]), "rdme_s": float(kv["rdme_s"])}
-    # the repository's FP64 restatement (single thread) when the reference
-    # build is not present
+            print(f"warning: the reference build failed (rc={r.returncode}); timing the FP64 oracle instead",
+                  file=sys.stderr)
+    else:
+        print(f"warning: {exe} is absent (built by `make -C oracle ref` where the reference sources are); "
+              "cpu_baseline times the FP64 oracle restatement instead (kind 'port')", file=sys.stderr)
     from oracle import oracle_py
     o = oracle_py.Oracle(sa, gauss_seidel=True)
     o.reset(seed=1)
@@ -94,7 +118,8 @@ def cpu_baseline(sa, steps, seconds_budget=30.0):
     o.advance(steps)
     sec = time.perf_counter() - t
     return {"value": sa.n * steps / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle restatement, 1 thread, {sa.n} particles, {steps} steps: {sec:.2f} s"}
+            "sample": f"FP64 oracle restatement, 1 thread, {sa.n} particles, steps 0..{steps - 1} "
+                      f"(the reference build was absent): {sec:.2f} s"}
 
 
 # matrix-core flops of one surrogate forward pass per node: encoder 32->64,
@@ -404,6 +429,112 @@ def bench_dambreak(args):
     print(json.dumps(out))
 
 
+def bench_cylinder(args):
+    """The 1M-particle 3D SDPD + sSSA cylinder: `settle` untimed steps, W
+    warm-up steps, K timed steps, then the rest of the 2,000-step run."""
+    from spatialpy_amd.engine import Engine
+    from spatialpy_amd import _abi
+    K = 20 if args.steps is None else args.steps
+    W = 5 if args.warmup is None else args.warmup
+    settle = args.settle
+    cpu_steps = 2 if args.cpu_steps is None else args.cpu_steps
+    first = settle + W
+    if first + K > CYL_RUN_STEPS:
+        print(f"the cylinder run is {CYL_RUN_STEPS} steps (settle {settle} + warmup {W} + steps {K} = "
+              f"{first + K}); choose a window inside it", file=sys.stderr)
+        sys.exit(2)
+    sa = build_workload(args.n, CYL_RUN_STEPS + 1)
+    if args.ann_order:
+        sa.flags |= _abi.FLAG_ANN_ORDER_MOVING
+    n = sa.n
+    eng = Engine(sa)
+    eng.reset(seed=12345)
+    t_run = time.perf_counter()
+    eng.advance(settle)
+    eng.advance(W)
+    pre_s = time.perf_counter() - t_run
+    # untimed: the state at the window's first step (the CPU sample starts
+    # from it) and the neighbour count per particle there
+    state = None if args.no_cpu_baseline else eng.state()
+    rp = eng.neighbors()[0]
+    E = float(np.diff(rp).mean())
+    st0, ph0 = eng.stats(), eng.phase_times()
+    t0 = time.perf_counter()
+    eng.advance(K)
+    t1 = time.perf_counter()
+    st1, ph1 = eng.stats(), eng.phase_times()
+    sec = t1 - t0
+    value = n * K / sec
+    events = (st1.reactions + st1.diffusions) - (st0.reactions + st0.diffusions)
+    dph = {k: (ph1[k][0] - ph0[k][0], ph1[k][1] - ph0[k][1]) for k in ph1}
+    full = None
+    if not args.no_full_run:
+        t2 = time.perf_counter()
+        eng.advance(CYL_RUN_STEPS - (first + K))
+        rest_s = time.perf_counter() - t2
+        st2 = eng.stats()
+        run_s = pre_s + sec + rest_s
+        full = {"steps": CYL_RUN_STEPS, "seconds": round(run_s, 3),
+                "particle_steps_per_s": n * CYL_RUN_STEPS / run_s,
+                "ssa_events": int(st2.reactions + st2.diffusions),
+                "ssa_events_per_s": (st2.reactions + st2.diffusions) / run_s,
+                "note": "the whole configured run, steps 0..1999 (host-side state and neighbour readbacks "
+                        "before the window excluded)"}
+
+    # roofline of the HBM-bound particle path (binning, search, SDPD
+    # substeps, rates and queue build)
+    B = algorithmic_bytes_per_particle_step(3, sa.n_species, E)
+    hbm_ms = dph["search"][0] + dph["sdpd"][0] + dph["forces"][0] + dph["rates_queue"][0] + dph["bin_tree"][0]
+    achieved = B * n * K / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
+    traffic = measured_traffic()
+    particle_roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic["particle_path"],
+                     "kernels": "binning + fixed-radius search + D_ij + SDPD substeps + rates/queue build",
+                     "algorithmic_bytes_per_particle_step": round(B, 1), "mean_degree": round(E, 3),
+                     "ms_per_step": round(hbm_ms / K, 4)}
+    # the SSA event loop: one dependent chain per trajectory; its algorithmic
+    # bytes are the voxel states an event reads and writes
+    ssa_ms = dph["ssa"][0]
+    n_re = st1.reactions - st0.reactions
+    n_di = st1.diffusions - st0.diffusions
+    ssa_roof = ssa_roofline(sa.n_species, sa.n_reactions, E, n_re, n_di, ssa_ms,
+                            "k_ssa (event loop; latency-bound, one wavefront per trajectory)")
+    ssa_roof["traffic"] = traffic["k_ssa"]
+    ssa_roof["ms_per_step"] = round(ssa_ms / K, 4)
+    dominant = "k_ssa" if ssa_ms > hbm_ms else "particle_path"
+    roofline = ssa_roof if dominant == "k_ssa" else particle_roof
+    out = {
+        "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": 1e3 * sec / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D SDPD+sSSA cylinder r=1 L=4, {n} particles, h=1.3*spacing, c0=20, "
+                               f"nu=1e-2, inflow 1.0, A+B->C k=1e-2 with {n} A + {n} B, dt={sa.dt:g}, "
+                               f"plain SDPD (no surrogate), seeded GPU stream; timed steps {first}..{first + K - 1} "
+                               f"of the {CYL_RUN_STEPS}-step run ({settle} settle + {W} warm-up steps first)",
+                   "particles": n, "dt": sa.dt, "run_steps": CYL_RUN_STEPS, "first_timed_step": first,
+                   "ann_order_moving": bool(args.ann_order)},
+        "ssa_events_per_s": events / sec,
+        "ssa_events": int(events),
+        "phase_ms_per_step": {k: round(v[0] / K, 4) for k, v in dph.items()},
+        "dominant": dominant,
+        "roofline": roofline,
+        "roofline_particle_path": particle_roof,
+        "roofline_ssa": ssa_roof,
+    }
+    if full is not None:
+        out["full_run"] = full
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_timing(eng, sa, dph["forces"][0] / K)
+    eng.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sa, state, first, cpu_steps)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        if "ssa_events_per_s" in out["cpu_baseline"]:
+            out["ssa_speedup_vs_cpu"] = out["ssa_events_per_s"] / out["cpu_baseline"]["ssa_events_per_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,12 +542,12 @@ def main():
                     help="cylinder: the 1M-particle moving SDPD+sSSA run (the headline line); static3d: the "
                          "64k-voxel static A+B<->C case, one trajectory and a seed batch; ensemble: uneven "
                          "gene-network models in one launch; dambreak: the 2D SDPD dam break with walls")
-    # the open cylinder at h = 1.3 x spacing, c0 = 20, dt = 2.5e-4 turns
-    # non-finite at step 21 in the reference engine itself (and identically
-    # here), so the default window stays inside steps 0..17
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (cylinder 15, static3d 2, dambreak 400)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (cylinder 2, static3d 0, "
-                                                              "dambreak 20)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (cylinder 20, static3d 2, dambreak 400)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (cylinder 5 after the settle "
+                                                              "steps, static3d 0, dambreak 20)")
+    ap.add_argument("--settle", type=int, default=CYL_SETTLE,
+                    help="cylinder: untimed steps before the warm-up (the timed window starts at step "
+                         "settle + warmup of the 2,000-step run)")
     ap.add_argument("--n", type=int, default=1_000_000, help="particles in the cylinder")
     ap.add_argument("--batch", type=int, default=256, help="static3d: trajectories in the seed batch")
     ap.add_argument("--models", type=int, default=1024, help="ensemble: number of models")
@@ -424,6 +555,8 @@ def main():
     ap.add_argument("--cpu-models", type=int, default=8, help="ensemble: models in the CPU baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline sample (cylinder 2, "
                                                                    "dambreak 20)")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="cylinder: stop after the timed window instead of finishing the 2,000-step run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate forward-pass timing")
@@ -453,89 +586,7 @@ def main():
         return bench_ensemble(args)
     if args.workload == "dambreak":
         return bench_dambreak(args)
-    args.steps = 15 if args.steps is None else args.steps
-    args.warmup = 2 if args.warmup is None else args.warmup
-    args.cpu_steps = 2 if args.cpu_steps is None else args.cpu_steps
-    if args.warmup + args.steps > 20:
-        print("warning: this workload turns non-finite at step 21 (also in the reference engine); "
-              "keep warmup + steps <= 20", file=sys.stderr)
-
-    from spatialpy_amd.engine import Engine
-    from spatialpy_amd import _abi
-
-    total_steps = args.warmup + args.steps
-    sa = build_workload(args.n, total_steps + 1)
-    if args.ann_order:
-        sa.flags |= _abi.FLAG_ANN_ORDER_MOVING
-    eng = Engine(sa)
-    eng.reset(seed=12345)
-    eng.advance(args.warmup)
-    st0 = eng.stats()
-    ph0 = eng.phase_times()
-    t0 = time.perf_counter()
-    eng.advance(args.steps)
-    t1 = time.perf_counter()
-    st1 = eng.stats()
-    ph1 = eng.phase_times()
-    sec = t1 - t0
-    n = sa.n
-    value = n * args.steps / sec
-    events = (st1.reactions + st1.diffusions) - (st0.reactions + st0.diffusions)
-    dph = {k: (ph1[k][0] - ph0[k][0], ph1[k][1] - ph0[k][1]) for k in ph1}
-
-    # roofline of the HBM-bound particle path (binning, search, SDPD
-    # substeps, rates and queue build)
-    E = 6
-    B = algorithmic_bytes_per_particle_step(3, sa.n_species, E)
-    hbm_ms = dph["search"][0] + dph["sdpd"][0] + dph["forces"][0] + dph["rates_queue"][0] + dph["bin_tree"][0]
-    achieved = B * n * args.steps / (hbm_ms * 1e-3) / 1e9 if hbm_ms > 0 else 0.0
-    traffic = measured_traffic()
-    particle_roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic["particle_path"],
-                     "kernels": "binning + fixed-radius search + D_ij + SDPD substeps + rates/queue build",
-                     "algorithmic_bytes_per_particle_step": B, "ms_per_step": round(hbm_ms / args.steps, 4)}
-    # the SSA event loop: one dependent chain per trajectory; its algorithmic
-    # bytes are the voxel states an event reads and writes
-    ssa_ms = dph["ssa"][0]
-    S_, R_ = sa.n_species, sa.n_reactions
-    b_react, b_diff = ssa_bytes_per_event(S_, R_, E)
-    n_re = st1.reactions - st0.reactions
-    n_di = st1.diffusions - st0.diffusions
-    ssa_bytes = b_react * n_re + b_diff * n_di
-    ssa_gbs = ssa_bytes / (ssa_ms * 1e-3) / 1e9 if ssa_ms > 0 else 0.0
-    ssa_roof = {"bound": "hbm", "achieved": round(ssa_gbs, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ssa_gbs / HBM_PEAK_GBS, 7), "traffic": traffic["k_ssa"],
-                "kernel": "k_ssa (event loop; latency-bound, one wavefront per trajectory)",
-                "algorithmic_bytes_per_event": {"reaction": b_react, "diffusion": b_diff},
-                "events": int(n_re + n_di), "us_per_event": round(1e3 * ssa_ms / max(n_re + n_di, 1), 3),
-                "ms_per_step": round(ssa_ms / args.steps, 4)}
-    dominant = "k_ssa" if ssa_ms > hbm_ms else "particle_path"
-    roofline = ssa_roof if dominant == "k_ssa" else particle_roof
-    out = {
-        "metric": "particle-steps/s", "value": value, "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"3D SDPD+sSSA cylinder r=1 L=4, {n} particles, h=1.3*spacing, c0=20, "
-                               f"nu=1e-2, inflow 1.0, A+B->C k=1e-2 with {n} A + {n} B, dt=2.5e-4, "
-                               f"plain SDPD (no surrogate), seeded GPU stream",
-                   "particles": n, "ann_order_moving": bool(args.ann_order)},
-        "ssa_events_per_s": events / sec,
-        "ssa_events": int(events),
-        "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in dph.items()},
-        "dominant": dominant,
-        "roofline": roofline,
-        "roofline_particle_path": particle_roof,
-        "roofline_ssa": ssa_roof,
-    }
-    if not args.no_surrogate:
-        out["surrogate"] = surrogate_timing(eng, sa, dph["forces"][0] / args.steps)
-    if not args.no_cpu_baseline:
-        from spatialpy_amd import workloads
-        csa = build_workload(args.n, args.cpu_steps + 1)
-        out["cpu_baseline"] = cpu_baseline(csa, args.cpu_steps)
-        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
-    print(json.dumps(out))
+    return bench_cylinder(args)
 
 
 if __name__ == "__main__":

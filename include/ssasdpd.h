@@ -234,6 +234,17 @@ int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *rng, ssasdpd_outputs *out)
 int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds,
                       ssasdpd_outputs *shared, uint32_t *counts);
 
+/* Wall-clock limit, in seconds, of each later ssasdpd_run, ssasdpd_run_batch
+ * and ssasdpd_advance call (0: none, the default).  The reference waits on
+ * the trajectory's process with Popen.wait(timeout), kills it when the limit
+ * passes and marks that result timed out (solver.py:574-586).  Here the call
+ * returns SSASDPD_ERR_TIMEOUT at the first step boundary past the limit: the
+ * snapshots taken before it are filled in, and the engine holds the state
+ * after the last completed step.  Static domains, whose remaining steps
+ * otherwise run in one launch, then run in launches of at most 64 steps
+ * (ending at snapshot steps), with the limit checked between them. */
+int ssasdpd_set_timeout(ssasdpd_engine *e, double seconds);
+
 /* Evaluate the pair-force kernel (pairwiseForce, model.cpp:39-191) once on
  * the engine's current state and neighbour lists without advancing:
  * F starts at gravity, Fbp and Frho at zero.  fp32 selects the FP32 pair

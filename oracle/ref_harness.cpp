@@ -276,6 +276,21 @@ int main(int argc, char **argv) {
         p->Frho = 0;
         p->bvf_phi = 0;
     }
+    /* continuing a run mid-way: the state after step start_step - 1 (the
+     * quantities take_step1 carries over from the previous step:
+     * simulate.cpp:68-83) */
+    const unsigned start_step = g_meta.count("start_step") ? geti("start_step") : 0;
+    if (g_meta.count("start_step")) {
+        auto F = load<double>("F.f64", 3 * (size_t)n), Fbp = load<double>("Fbp.f64", 3 * (size_t)n);
+        auto Frho = load<double>("Frho.f64", n);
+        auto Cst = load<double>("C.f64", (size_t)n * Sc), Qst = load<double>("Q.f64", (size_t)n * Sc);
+        for (int i = 0; i < n; i++) {
+            Particle &p = system->particles[i];
+            for (int k = 0; k < 3; k++) { p.F[k] = F[3 * i + k]; p.Fbp[k] = Fbp[3 * i + k]; }
+            p.Frho = Frho[i];
+            for (int s = 0; s < Sc; s++) { p.C[s] = Cst[(size_t)i * Sc + s]; p.Q[s] = Qst[(size_t)i * Sc + s]; }
+        }
+    }
     if (S > 0) initialize_rdme(system, irN.data(), jcN.data(), prN.data(), irG.data(), jcG.data(), u0.data());
 
     std::vector<uint32_t> snap_xx;
@@ -342,18 +357,18 @@ int main(int argc, char **argv) {
         const unsigned steps = run_steps;
         pthread_barrier_t bar;
         pthread_barrier_init(&bar, nullptr, threads + 1);
-        struct Ctx { ParticleSystem *sys; int first, count; pthread_barrier_t *bar; unsigned steps; };
+        struct Ctx { ParticleSystem *sys; int first, count; pthread_barrier_t *bar; unsigned first_step, steps; };
         std::vector<Ctx> ctx(threads);
         std::vector<pthread_t> th(threads);
         int per = n / threads, left = n, first = 0;
         for (int t = 0; t < threads; t++) {
             int cnt = (t == threads - 1) ? left : per;
-            ctx[t] = Ctx{system, first, cnt, &bar, steps};
+            ctx[t] = Ctx{system, first, cnt, &bar, start_step, steps};
             first += cnt;
             left -= cnt;
             pthread_create(&th[t], nullptr, [](void *arg) -> void * {
                 Ctx *c = (Ctx *)arg;
-                for (unsigned step = 0; step < c->steps; step++) {
+                for (unsigned step = c->first_step; step < c->first_step + c->steps; step++) {
                     pthread_barrier_wait(c->bar);
                     for (unsigned sub = 0; sub < 3; sub++) {
                         for (int i = 0; i < c->count; i++)
@@ -367,7 +382,7 @@ int main(int argc, char **argv) {
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
         double kd_s = 0, rdme_s = 0;
-        for (system->current_step = 0; system->current_step < steps; system->current_step++) {
+        for (system->current_step = start_step; system->current_step < start_step + steps; system->current_step++) {
             struct timespec a, b;
             clock_gettime(CLOCK_MONOTONIC, &a);
             buildKDTree(system);
@@ -383,13 +398,14 @@ int main(int argc, char **argv) {
         clock_gettime(CLOCK_MONOTONIC, &t1);
         for (int t = 0; t < threads; t++) pthread_join(th[t], nullptr);
         double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
-        printf("REF_TIMING seconds=%.6f steps=%u n=%d reactions=%ld diffusions=%ld threads=%d kd_s=%.6f rdme_s=%.6f\n",
-               sec, steps, n, system->total_reactions, system->total_diffusion, threads, kd_s, rdme_s);
+        printf("REF_TIMING seconds=%.6f steps=%u n=%d reactions=%ld diffusions=%ld threads=%d kd_s=%.6f rdme_s=%.6f "
+               "start_step=%u\n",
+               sec, steps, n, system->total_reactions, system->total_diffusion, threads, kd_s, rdme_s, start_step);
         fflush(stdout);
         _exit(0);
     }
     unsigned next_output = 0;
-    for (system->current_step = 0; system->current_step < run_steps; system->current_step++) {
+    for (system->current_step = start_step; system->current_step < start_step + run_steps; system->current_step++) {
         unsigned step = system->current_step;
         buildKDTree(system);
         if (step >= next_output) {

@@ -78,7 +78,8 @@ class Reaction:
         self.annotation = annotation
         if restrict_to is not None and not isinstance(restrict_to, list):
             restrict_to = [restrict_to]
-        self.restrict_to = restrict_to
+        # stored as type names, as reaction.py:136-139 stores them
+        self.restrict_to = None if restrict_to is None else [f"type_{t}" for t in restrict_to]
         if rate is not None:
             self.massaction = True
             self.type = "mass-action"

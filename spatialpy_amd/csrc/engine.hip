@@ -70,6 +70,10 @@ double kernel_alpha(int dim, double h) {
 
 }  // namespace
 
+// With a time limit set, a fused static-domain run launches at most this
+// many steps at a time and checks the limit between launches.
+constexpr uint32_t FUSED_SEGMENT = 64;
+
 struct ssasdpd_engine {
     // configuration
     int n = 0, dim = 3, T = 1, S = 0, R = 0, Sc = 0, Rc = 0, static_domain = 1, nbc = 0;
@@ -100,6 +104,26 @@ struct ssasdpd_engine {
     hipStream_t ps = nullptr;
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
+    // Wall-clock limit of one run / run_batch / advance call (0: none),
+    // checked between steps and between the segments of a fused static run.
+    double timeout_s = 0;
+    bool deadline_on = false;
+    std::chrono::steady_clock::time_point deadline{};
+    void arm_deadline() {
+        deadline_on = timeout_s > 0;
+        if (deadline_on)
+            deadline = std::chrono::steady_clock::now() +
+                       std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                           std::chrono::duration<double>(timeout_s));
+    }
+    // SSASDPD_ERR_TIMEOUT once the deadline has passed (the state is that
+    // after step cur - 1), else 0.
+    int deadline_rc() const {
+        if (!deadline_on || std::chrono::steady_clock::now() < deadline) return 0;
+        char buf[96];
+        snprintf(buf, sizeof buf, "timeout after %.3f s: stopped before step %u", timeout_s, cur);
+        return set_error(SSASDPD_ERR_TIMEOUT, buf);
+    }
     int *deg_alt = nullptr;
     EdgeRec *erow_alt = nullptr;
     // binning / tree
@@ -846,6 +870,11 @@ struct ssasdpd_engine {
             la_pending = true;
         }
         int rc = check(step, !la);
+        if (rc && la) {
+            // the next step's particle phase is still running on ps: let it
+            // finish so that state reads on st do not race it
+            HIP_OK(hipStreamSynchronize(ps));
+        }
         end_step();
         return rc;
     }
@@ -1248,14 +1277,19 @@ int ssasdpd_advance(ssasdpd_engine *e, uint32_t n_steps) {
     GUARD_BEGIN
     auto t0 = std::chrono::steady_clock::now();
     const uint32_t target = (uint32_t)std::min<uint64_t>((uint64_t)e->cur + n_steps, e->nt);
+    e->arm_deadline();
     while (e->cur < target) {
+        if (int rc = e->deadline_rc()) return rc;
         if (e->fusable()) {
-            int rc = e->particle_steps(e->cur, target, {}, [](size_t) {});
+            // with a time limit, fused launches of at most FUSED_SEGMENT
+            // steps, the limit checked between them
+            const uint32_t seg = e->deadline_on ? std::min<uint32_t>(target, e->cur + FUSED_SEGMENT) : target;
+            int rc = e->particle_steps(e->cur, seg, {}, [](size_t) {});
             if (rc) return rc;
-            e->ssa_fused(e->rdme_args(e->cur), 1, target, {}, nullptr, 0, nullptr);
-            rc = e->check(target - 1);
+            e->ssa_fused(e->rdme_args(e->cur), 1, seg, {}, nullptr, 0, nullptr);
+            rc = e->check(seg - 1);
             if (rc) return rc;
-            break;
+            continue;
         }
         int rc = e->do_step(e->cur + 1 < target);
         if (rc) return rc;
@@ -1288,11 +1322,33 @@ static std::vector<unsigned> plan_snapshots(const ssasdpd_engine *e, uint32_t s0
 int ssasdpd_run(ssasdpd_engine *e, const ssasdpd_rng *r, ssasdpd_outputs *o) {
     GUARD_BEGIN
     e->reset(r);
+    e->arm_deadline();
     auto t0 = std::chrono::steady_clock::now();
     uint32_t next = 0;
     size_t idx = 0;
     int k = 0;
     for (uint32_t step = 0; step < e->nt; step++) {
+        if (int rc = e->deadline_rc()) return rc;
+        if (e->fusable() && e->deadline_on) {
+            // with a time limit: the snapshot due before this step, then the
+            // steps up to the next snapshot (at most FUSED_SEGMENT) in one
+            // launch, the limit checked between launches
+            if (step >= next) {
+                if (o && k < o->n_snapshots) e->snapshot(o, k);
+                k++;
+                next = idx < e->out_steps.size() ? e->out_steps[idx] : 0xFFFFFFFFu;
+                idx++;
+            }
+            // the next snapshot is due before step max(next, step + 1)
+            const uint32_t seg = std::min<uint32_t>({e->nt, std::max<uint32_t>(next, step + 1), step + FUSED_SEGMENT});
+            int rc = e->particle_steps(step, seg, {}, [](size_t) {});
+            if (rc) return rc;
+            e->ssa_fused(e->rdme_args(step), 1, seg, {}, nullptr, 0, nullptr);
+            rc = e->check(seg - 1);
+            if (rc) return rc;
+            step = seg - 1;
+            continue;
+        }
         if (e->fusable()) {
             // all remaining steps in one launch, count snapshots in-kernel
             std::vector<int> ks;
@@ -1352,6 +1408,7 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
     }
 
     e->batch_reset(B, seeds);
+    e->arm_deadline();
     auto t0 = std::chrono::steady_clock::now();
     const size_t per = (size_t)e->n * e->S;
     auto snap_shared = [&](int k, uint32_t step) {
@@ -1375,6 +1432,26 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
     size_t idx = 0;
     int k = 0;
     for (uint32_t step = 0; step < e->nt; step++) {
+        if (int rc = e->deadline_rc()) return rc;
+        if (e->fusable_static() && e->batch_all_init() && e->deadline_on) {
+            // as in ssasdpd_run: segments between snapshots, limit checked
+            // between launches
+            if (step >= next) {
+                if (k < n_snap) snap(k);
+                k++;
+                next = idx < e->out_steps.size() ? e->out_steps[idx] : 0xFFFFFFFFu;
+                idx++;
+            }
+            // the next snapshot is due before step max(next, step + 1)
+            const uint32_t seg = std::min<uint32_t>({e->nt, std::max<uint32_t>(next, step + 1), step + FUSED_SEGMENT});
+            int rc = e->particle_steps(step, seg, {}, [](size_t) {});
+            if (rc) return rc;
+            e->ssa_fused(e->batch_args(step, -1), B, seg, {}, nullptr, 0, nullptr);
+            rc = e->batch_check(B, seg - 1);
+            if (rc) return rc;
+            step = seg - 1;
+            continue;
+        }
         if (e->fusable_static() && e->batch_all_init()) {
             std::vector<int> ks;
             std::vector<unsigned> sst = plan_snapshots(e, step, next, idx, k, ks);
@@ -2007,6 +2084,15 @@ int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m, 
     dfree(kd.parent); dfree(kd.depth); dfree(kd.cd); dfree(kd.cv); dfree(kd.is_hi); dfree(kd.leaf_of);
     dfree(de); dfree(oi); dfree(orr); dfree(dg);
     (void)hipStreamDestroy(st);
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_set_timeout(ssasdpd_engine *e, double seconds) {
+    GUARD_BEGIN
+    if (!(seconds >= 0) || !std::isfinite(seconds))
+        return set_error(SSASDPD_ERR_BAD_DESC, "set_timeout: seconds must be finite and >= 0");
+    e->timeout_s = seconds;
     return 0;
     GUARD_END
 }

@@ -76,6 +76,12 @@ class Engine:
         r, self._keep = make_rng(seed, words, expo)
         _abi.check(self.lib, self.lib.ssasdpd_reset(self.h, C.byref(r)), "ssasdpd_reset")
 
+    def set_timeout(self, seconds):
+        """Wall-clock limit of each later run / run_batch / advance call
+        (None or 0: none); past it the call raises SimulationError with code
+        SSASDPD_ERR_TIMEOUT at a step boundary."""
+        _abi.check(self.lib, self.lib.ssasdpd_set_timeout(self.h, float(seconds or 0.0)), "set_timeout")
+
     def advance(self, n):
         _abi.check(self.lib, self.lib.ssasdpd_advance(self.h, n), "ssasdpd_advance")
 
@@ -143,8 +149,11 @@ class Engine:
                    "ssasdpd_eval_forces")
         return F, Fbp, Frho
 
-    def run(self, seed=None, words=None, expo=None, fields=("xx", "C", "x", "v", "rho")):
-        """Full trajectory with the reference's snapshot schedule."""
+    def run(self, seed=None, words=None, expo=None, fields=("xx", "C", "x", "v", "rho"), partial_ok=False):
+        """Full trajectory with the reference's snapshot schedule.  With
+        partial_ok, a run stopped by the time limit (set_timeout) returns the
+        snapshots taken before it and res["timeout"] = True instead of
+        raising."""
         sa = self.sa
         steps = output_schedule(sa.output_steps, sa.nt)
         k = steps.size
@@ -169,10 +178,13 @@ class Engine:
         res["steps"] = np.zeros(k, np.uint32)
         o.steps = _p(res["steps"], C.c_uint32)
         r, keep = make_rng(seed, words, expo)
-        _abi.check(self.lib, self.lib.ssasdpd_run(self.h, C.byref(r), C.byref(o)), "ssasdpd_run")
+        rc = self.lib.ssasdpd_run(self.h, C.byref(r), C.byref(o))
+        res["timeout"] = partial_ok and rc == _abi.TIMEOUT
+        if not res["timeout"]:
+            _abi.check(self.lib, rc, "ssasdpd_run")
         return res
 
-    def run_batch(self, seeds, fields=("xx", "C", "x", "v", "rho")):
+    def run_batch(self, seeds, fields=("xx", "C", "x", "v", "rho"), partial_ok=False):
         """Trajectories seeds[0..B) of the model in one batched run (seeded
         GPU streams).  Returns a dict with "xx" of shape [B][K][n][S] and the
         shared particle fields of shape [K][...] (the particle state does not
@@ -200,8 +212,10 @@ class Engine:
         res["steps"] = np.zeros(k, np.uint32)
         o.steps = _p(res["steps"], C.c_uint32)
         counts = np.zeros((B, k, n, self.S), np.uint32)
-        _abi.check(self.lib, self.lib.ssasdpd_run_batch(self.h, B, _p(seeds, C.c_uint64), C.byref(o),
-                                                        _p(counts, C.c_uint32)), "ssasdpd_run_batch")
+        rc = self.lib.ssasdpd_run_batch(self.h, B, _p(seeds, C.c_uint64), C.byref(o), _p(counts, C.c_uint32))
+        res["timeout"] = partial_ok and rc == _abi.TIMEOUT
+        if not res["timeout"]:
+            _abi.check(self.lib, rc, "ssasdpd_run_batch")
         res["xx"] = counts
         return res
 

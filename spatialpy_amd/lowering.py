@@ -257,8 +257,13 @@ def output_schedule(output_steps, nt):
 
 
 def write_harness_dir(sa: SystemArrays, path, seed, run_steps=None, debug_flag=0,
-                      force_step=0, nbr_step=0, threads=0, write_vtk=0):
-    """Write a model in the raw format read by oracle/ref_harness.cpp."""
+                      force_step=0, nbr_step=0, threads=0, write_vtk=0, state=None, start_step=0):
+    """Write a model in the raw format read by oracle/ref_harness.cpp.
+
+    state / start_step: continue a run mid-way.  state holds the particle
+    state after step start_step - 1 (x, v, rho, F, Fbp, Frho, C, Q and the
+    counts xx, as Engine.state() returns them); the harness then runs steps
+    start_step .. start_step + run_steps - 1."""
     sa.normalized()
     os.makedirs(path, exist_ok=True)
     S = sa.n_species if sa.enable_rdme else 0
@@ -281,19 +286,29 @@ def write_harness_dir(sa: SystemArrays, path, seed, run_steps=None, debug_flag=0
     }
     if run_steps is not None:
         meta["run_steps"] = run_steps
+    if state is not None:
+        meta["start_step"] = int(start_step)
     with open(os.path.join(path, "meta.txt"), "w") as f:
         for k, v in meta.items():
             f.write(f"{k} {v}\n")
     w = lambda name, a, dt: np.ascontiguousarray(a, dtype=dt).tofile(os.path.join(path, name))
-    w("x.f64", sa.x, "<f8")
-    w("v.f64", sa.v if sa.v is not None else np.zeros((sa.n, 3)), "<f8")
+    w("x.f64", sa.x if state is None else state["x"], "<f8")
+    w("v.f64", (sa.v if sa.v is not None else np.zeros((sa.n, 3))) if state is None else state["v"], "<f8")
+    if state is not None:
+        w("F.f64", state["F"], "<f8")
+        w("Fbp.f64", state["Fbp"], "<f8")
+        w("Frho.f64", state["Frho"], "<f8")
+        w("Q.f64", state["Q"], "<f8")
+        w("C.f64", state["C"], "<f8")
     w("type.i32", sa.type, "<i4")
     w("nu.f64", sa.nu, "<f8")
     w("mass.f64", sa.mass, "<f8")
-    w("rho.f64", sa.rho, "<f8")
+    w("rho.f64", sa.rho if state is None else state["rho"], "<f8")
     w("fixed.i32", sa.fixed, "<i4")
     nsp = sa.n_species
     u0 = sa.u0 if sa.u0 is not None else np.zeros((sa.n, 0), np.uint32)
+    if state is not None and nsp:
+        u0 = state["xx"]
     w("u0.u32", u0, "<u4")
     dm = sa.diffusion_matrix if sa.diffusion_matrix is not None else np.zeros((0,))
     w("dmat.f64", dm, "<f8")

@@ -4,7 +4,6 @@ through the C ABI instead of generating, compiling and spawning a C++
 program."""
 import math
 import os
-import time
 
 import numpy as np
 
@@ -129,7 +128,18 @@ class Solver:
     def lower(self):
         """The model as engine arrays (no GPU needed)."""
         m = self.model
-        params = m.compile_prep()
+        prep = m.compile_prep()
+        if isinstance(prep, tuple):
+            # a spatialpy.Model: compile_prep returns the stoichiometry and
+            # dependency graph and resolves each Parameter's value in place
+            # (core/model.py:982-1019)
+            params = {name: float(p.value) for name, p in m.listOfParameters.items()}
+        else:
+            params = prep
+        # parameter names as the generated code spells them (P0, P1, ...;
+        # core/model.py:703-715): compile_prep may leave a species' diffusion
+        # coefficient as one of these
+        sanitized = {f"P{i}": name for i, name in enumerate(m.listOfParameters)}
         d = m.domain
         types = d.typeNdxMapping
         T = len(types) - 1
@@ -139,15 +149,15 @@ class Solver:
         sidx = {s.name: i for i, s in enumerate(species)}
 
         def sindex(x):
-            return sidx[x.name if isinstance(x, Species) else x]
+            return sidx[x if isinstance(x, str) else x.name]
 
         # diffusion matrix [S][T] (solver.py:265-286)
         D = np.zeros((S, T))
         for i, sp in enumerate(species):
             coef = sp.diffusion_coefficient
             if isinstance(coef, str):
-                coef = params[coef]
-            elif isinstance(coef, Parameter):
+                coef = params[sanitized.get(coef, coef)]
+            elif isinstance(coef, Parameter) or hasattr(coef, "value"):
                 coef = coef.value
             for name, j in types.items():
                 if j == 0:
@@ -186,7 +196,7 @@ class Solver:
             mask = 0
             if r.restrict_to:
                 for t in r.restrict_to:
-                    mask |= 1 << types[f"type_{t}"]
+                    mask |= 1 << types[t]
             rx.append((kind, a, b, k, mask, reac, prod))
         x = d.vertices
         if not np.count_nonzero(x[:, 1]):
@@ -202,21 +212,25 @@ class Solver:
             raise ModelError("h (basis function width) can not be zero.")
         bcs = []
         for bc in m.listOfBoundaryConditions:
-            lo = [-math.inf if v is None else float(v) for v in bc.bounds[0::2]]
-            hi = [math.inf if v is None else float(v) for v in bc.bounds[1::2]]
+            bounds = getattr(bc, "bounds", None)
+            if bounds is None:      # a spatialpy.BoundaryCondition
+                bounds = (bc.xmin, bc.xmax, bc.ymin, bc.ymax, bc.zmin, bc.zmax)
+            lo = [-math.inf if v is None else float(v) for v in bounds[0::2]]
+            hi = [math.inf if v is None else float(v) for v in bounds[1::2]]
             tt = -1 if bc.type_id is None else types[bc.type_id]
-            if bc.target in m.listOfSpecies:
+            target = bc.target if isinstance(bc.target, str) else bc.target.name
+            if target in m.listOfSpecies:
                 if not bc.deterministic:
                     raise SimulationError("BoundaryConditions don't work for stochastic species yet.")
-                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_CHEM, species=sidx[bc.target],
+                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_CHEM, species=sidx[target],
                                 value=[float(bc.value)]))
-            elif bc.target == "v":
+            elif target == "v":
                 bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_V, value=[float(v) for v in bc.value]))
-            elif bc.target in ("nu", "rho"):
-                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_NU if bc.target == "nu" else _abi.BC_RHO,
+            elif target in ("nu", "rho"):
+                bcs.append(dict(lo=lo, hi=hi, type=tt, target=_abi.BC_NU if target == "nu" else _abi.BC_RHO,
                                 value=[float(bc.value)]))
             else:
-                raise SimulationError(f"Unable handle boundary condition for property '{bc.target}'")
+                raise SimulationError(f"Unable handle boundary condition for property '{target}'")
         ts = m.tspan
         g = tuple(d.gravity) if d.gravity is not None else (0.0, 0.0, 0.0)
         sa = SystemArrays(
@@ -243,45 +257,47 @@ class Solver:
 
     def run(self, number_of_trajectories=1, seed=None, timeout=None, number_of_threads=None, debug=False,
             profile=False, verbose=True):
-        """solver.py:510-607: trajectory i uses seed+i."""
+        """solver.py:510-607: trajectory i uses seed+i.  timeout applies to
+        each trajectory as the reference's Popen.wait(timeout) does
+        (solver.py:574-586): a trajectory that runs past it stops at the next
+        step boundary, keeps the snapshots taken so far and is marked
+        result.timeout; the remaining trajectories still run.  A batched run
+        (several seeded trajectories at once) shares one limit, since its
+        trajectories advance together."""
         if not self.is_compiled:
             self.compile(debug=debug, profile=profile)
+        self.engine.set_timeout(timeout)
+        fields = ("xx", "C", "x", "v", "rho", "bvf_phi", "nu")
         first = None
-        t0 = time.monotonic()
         if number_of_trajectories > 1 and not self.exact_replay and self.sa.n_species > 0:
             # one batched GPU run: the particle state is shared, every
             # trajectory has its own counts, queue and stream (seed + i)
             base = seed if seed is not None else int.from_bytes(os.urandom(8), "little")
             seeds = [(base + i) & 0xFFFFFFFFFFFFFFFF for i in range(number_of_trajectories)]
-            fields = ("xx", "C", "x", "v", "rho", "bvf_phi", "nu")
-            data = self.engine.run_batch(seeds, fields=fields)
+            data = self.engine.run_batch(seeds, fields=fields, partial_ok=True)
             for i in range(number_of_trajectories):
                 d = {f: data[f] for f in fields if f != "xx" and f in data}
                 d["xx"] = data["xx"][i]
                 d["steps"] = data["steps"]
                 res = Result(self.model, d, d["steps"], self.sa)
+                res.timeout = bool(data["timeout"])
                 if first is None:
                     first = res
                 else:
                     first.append(res)
-            if timeout is not None and time.monotonic() - t0 > timeout:
-                first.timeout = True
             return first
         for i in range(number_of_trajectories):
             s = (seed + i) if seed is not None else int.from_bytes(os.urandom(8), "little")
             if self.exact_replay:
                 data = self._run_replay(s)
             else:
-                data = self.engine.run(seed=s, fields=("xx", "C", "x", "v", "rho", "bvf_phi", "nu"))
+                data = self.engine.run(seed=s, fields=fields, partial_ok=True)
             res = Result(self.model, data, data["steps"], self.sa)
-            if timeout is not None and time.monotonic() - t0 > timeout:
-                res.timeout = True
+            res.timeout = bool(data.pop("timeout", False))
             if first is None:
                 first = res
             else:
                 first.append(res)
-            if res.timeout:
-                break
         return first
 
     def _run_replay(self, seed):
@@ -290,7 +306,8 @@ class Solver:
         while True:
             words, expo = mt19937_64_stream(seed & 0xFFFFFFFFFFFFFFFF, n)
             try:
-                return self.engine.run(words=words, expo=expo, fields=("xx", "C", "x", "v", "rho", "bvf_phi", "nu"))
+                return self.engine.run(words=words, expo=expo, fields=("xx", "C", "x", "v", "rho", "bvf_phi", "nu"),
+                                       partial_ok=True)
             except SimulationError as e:
                 if e.code != 9 or n >= (1 << 31):
                     raise

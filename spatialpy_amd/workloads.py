@@ -187,8 +187,25 @@ def dam_break_2d(nx=200, ny=200, spacing=0.0025, box=(2.0, 1.5), n_mol=1000, D=1
     return sa.normalized()
 
 
+# Time step of the 1M-particle cylinder.  The acoustic limit is
+# 0.25 h / c0 = 3.8e-4 at h = 1.3 x 0.023249; at 2.5e-4 the open cylinder
+# turns non-finite at step 21 (reference engine, oracle and this engine
+# alike), while at 1.25e-4 the 1M run and the oracle's 20k-particle run at
+# the same Courant number (dt scaled with the spacing) both stay finite over
+# 2,000 steps.
+CYLINDER_DT_1M = 1.25e-4
+
+
+def cylinder_dt(n_target, radius=1.0, length=4.0):
+    """CYLINDER_DT_1M scaled with the lattice spacing to n_target particles
+    (the same Courant number c0 dt / h at every size)."""
+    s = (math.pi * radius * radius * length / n_target) ** (1.0 / 3.0)
+    s1m = (math.pi * radius * radius * length / 1_000_000) ** (1.0 / 3.0)
+    return CYLINDER_DT_1M * s / s1m
+
+
 def cylinder_3d(n_target=1_000_000, radius=1.0, length=4.0, rho0=1.0, c0=20.0, nu=1e-2,
-                inflow=1.0, nA=None, nB=None, k=1e-2, D=1e-3, dt=2.5e-4, nt=2000,
+                inflow=1.0, nA=None, nB=None, k=1e-2, D=1e-3, dt=None, nt=2000,
                 out_every=None, seed=0, lattice_spacing=None, walls=0, h_factor=1.3):
     """Config 5: fluid filling a cylinder (axis x) on a cubic lattice with
     spacing (pi r^2 L / n_target)^(1/3), inflow v=(inflow,0,0) on the slab
@@ -196,7 +213,9 @@ def cylinder_3d(n_target=1_000_000, radius=1.0, length=4.0, rho0=1.0, c0=20.0, n
     of fixed wall particles (type 2) around the fluid, radius in
     (radius, radius + walls*spacing], so the fluid is a pipe flow rather than
     a free jet (the SDPD's background-pressure transport term pushes a free
-    surface outward)."""
+    surface outward).  dt=None: cylinder_dt(n_target)."""
+    if dt is None:
+        dt = cylinder_dt(n_target, radius, length)
     vol_total = math.pi * radius * radius * length
     s = lattice_spacing or (vol_total / n_target) ** (1.0 / 3.0)
     nxs = int(math.floor(length / s)) + 1

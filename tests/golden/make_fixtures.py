@@ -36,7 +36,7 @@ CASES = {
                                params=dict(nx=20, ny=20, spacing=0.025, n_mol=100, nt=45, out_every=15, dt=2e-4),
                                seed=5, kw=dict(force_step=25, nbr_step=25)),
     "c5_cylinder_small": dict(gen="cylinder_3d",
-                              params=dict(n_target=1200, nt=10, out_every=5, seed=4),
+                              params=dict(n_target=1200, nt=10, out_every=5, seed=4, dt=2.5e-4),
                               seed=9, kw=dict(force_step=6, nbr_step=6)),
     # custom propensity functions (the reference compiles their generated
     # C++; the harness loads it from props.so)

@@ -1,5 +1,6 @@
-"""The 1M-particle cylinder (A+B->C in a 3D SDPD pipe inflow, 1,005,649
-particles) at its full size, through properties that hold at any size:
+"""The 1M-particle cylinder (A+B->C in a 3D SDPD free jet: fluid fed at 1.0
+along the axis, no walls; 1,005,649 particles) at its full size, through
+properties that hold at any size:
 
 * fixed-radius search (particle.cpp:240-294): the neighbour set of every
   particle equals scipy's cKDTree pair set within h (an independent
@@ -9,7 +10,10 @@ particles) at its full size, through properties that hold at any size:
   obey A = A0 - R, B = B0 - R and C = R, with R the engine's reaction count;
 * the seeded (counter-based) stream makes a run repeatable: two engines on
   the same seed end in the same state, bit for bit, whether the steps are
-  taken in one advance() call or two."""
+  taken in one advance() call or two;
+* at its time step (1.25e-4) the run completes all 2,000 configured steps
+  (the reference's step loop, simulate_threads.cpp:232-281) with finite
+  state and the A+B->C totals intact."""
 import numpy as np
 import pytest
 from scipy.spatial import cKDTree
@@ -83,3 +87,20 @@ def test_full_size_cylinder_conserves_molecules_and_repeats(cylinder):
     assert np.array_equal(st["xx"], st2["xx"])
     for f in ("x", "v", "rho"):
         assert np.array_equal(st[f].view(np.uint64), st2[f].view(np.uint64)), f
+
+
+def test_full_size_cylinder_runs_the_configured_2000_steps():
+    sa = workloads.cylinder_3d(n_target=1_000_000, nt=2001)
+    assert sa.n == N_FULL and sa.dt == workloads.CYLINDER_DT_1M
+    A0, B0 = (int(v) for v in sa.u0[:, :2].astype(np.int64).sum(axis=0))
+    eng = Engine(sa)
+    eng.reset(seed=12345)
+    for k in range(4):
+        eng.advance(500)
+        st = eng.state()
+        R = int(eng.stats().reactions)
+        t = st["xx"].astype(np.int64).sum(axis=0)
+        assert (int(t[0]), int(t[1]), int(t[2])) == (A0 - R, B0 - R, R), k
+        assert np.isfinite(st["x"]).all() and np.isfinite(st["v"]).all() and np.isfinite(st["rho"]).all(), k
+        assert 0.2 < st["rho"].min() and st["rho"].max() < 5.0, (k, st["rho"].min(), st["rho"].max())
+    assert eng.stats().steps == 2000
