@@ -70,6 +70,9 @@ double kernel_alpha(int dim, double h) {
 
 }  // namespace
 
+// The calling thread's error message, for the other translation units.
+int ssa::report_error(int code, const std::string &msg) { return set_error(code, msg); }
+
 // With a time limit set, a fused static-domain run launches at most this
 // many steps at a time and checks the limit between launches.
 constexpr uint32_t FUSED_SEGMENT = 64;

@@ -30,6 +30,9 @@ struct HipError {
     }
 };
 
+// Record msg as the calling thread's ssasdpd_last_error() and return code.
+int report_error(int code, const std::string &msg);
+
 constexpr int WAVE = 64;
 constexpr int TREE_FAN = 64;
 

@@ -57,3 +57,56 @@ def test_ensemble_rejects_mixed_schedules_and_moving_domains():
     mv = workloads.dam_break_2d(nx=10, ny=10, spacing=0.05, n_mol=20, nt=4, out_every=2, dt=2e-4)
     with pytest.raises(SimulationError, match="static"):
         Ensemble([mv])
+
+
+def test_gene_network_replay_matches_the_oracle_event_by_event(oracle):
+    """A gene-network model (5 species, 6 mass-action reactions, a 2,744-voxel
+    jittered lattice) in replay mode: the engine's event sequence, counters
+    and count snapshots equal the FP64 oracle's (the reference's event loop,
+    simulate_rdme.cpp:211-472) on the same std::mt19937_64 stream, and every
+    one of the six reactions fires."""
+    from spatialpy_amd.engine import mt19937_64_stream
+    sa = workloads.gene_network_3d(n_target=3000, seed=21, t_end=6)
+    cap = 200_000
+    words, expo = mt19937_64_stream(5, 1_000_000)
+    eng = Engine(sa, trace_capacity=cap)
+    res = eng.run(words=words, expo=expo, fields=("xx",))
+    ev, n = eng.trace(cap)
+    o = oracle.Oracle(sa)
+    o.set_trace(cap)
+    ref = o.run(seed=5)
+    oev, on = o.trace(cap)
+    assert n == on and 1000 < n < cap
+    assert np.array_equal(ev, oev)
+    assert np.array_equal(res["xx"], ref["xx"])
+    st, ost = eng.stats(), o.stats()
+    assert (st.reactions, st.diffusions, st.rng_words_used) == (ost.reactions, ost.diffusions, ost.rng_words_used)
+    fired = np.bincount(ev["which"][ev["kind"] == 0], minlength=6)
+    assert (fired > 0).all(), fired
+
+
+def test_configured_size_ensemble_equals_per_trajectory_runs():
+    """The batched-ensemble configuration at its size: 64 gene-network models
+    with N ~ U[8000, 64000] voxels in one launch (t = 0..2).  Per model:
+    totals equal the plain single-model GPU run with the same seed (hence
+    the ensemble mean and variance over trajectories equal the per-trajectory
+    runs' at every output time), the first snapshot holds u0, and the gene
+    count G_on + G_off -- conserved by all six reactions -- never changes."""
+    rng = np.random.default_rng(7)
+    sizes = rng.integers(8000, 64001, size=64)
+    models = [workloads.gene_network_3d(n_target=int(nv), seed=500 + i, t_end=2) for i, nv in enumerate(sizes)]
+    assert min(m.n for m in models) >= 6859 and max(m.n for m in models) <= 64000
+    seeds = [9000 + i for i in range(len(models))]
+    ens = Ensemble(models)
+    totals = ens.run(seeds).astype(np.int64)
+    st = ens.stats()
+    assert st.reactions > 0 and st.diffusions > 0
+    single = np.zeros_like(totals)
+    for m, (sa, s) in enumerate(zip(models, seeds)):
+        single[m] = Engine(sa).run(seed=s, fields=("xx",))["xx"].astype(np.int64).sum(axis=1)
+        assert np.array_equal(totals[m, 0], sa.u0.astype(np.int64).sum(axis=0)), m
+        genes = totals[m, :, 0] + totals[m, :, 1]
+        assert (genes == genes[0]).all(), (m, genes)
+    assert np.array_equal(totals, single)
+    assert np.array_equal(totals.mean(axis=0), single.mean(axis=0))
+    assert np.array_equal(totals.var(axis=0), single.var(axis=0))
