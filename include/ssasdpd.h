@@ -378,6 +378,26 @@ int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m,
                       int32_t dim, double h, int64_t *row_ptr, int32_t *idx,
                       double *r2);
 
+/* ssasdpd_fr_search without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and two-call protocol as ssasdpd_fr_search; every device buffer (staging
+ * of pts and q, cell binning, kd-tree and its work space, lists, error
+ * record) lives in one file-scope device array of about 45 MiB that each
+ * GPU holds until the process ends.  Limits: n <= SSASDPD_NOALLOC_MAX_POINTS,
+ * m <= SSASDPD_NOALLOC_MAX_QUERIES (larger inputs are refused with
+ * SSASDPD_ERR_CAPACITY before any GPU work), and at most
+ * SSASDPD_NOALLOC_MAX_WIDTH neighbours per query (a wider list is reported
+ * as SSASDPD_ERR_CAPACITY after the search).  Calls on one GPU share that
+ * array and each overwrites it: calls that overlap in time race, so callers
+ * serialise them. */
+#define SSASDPD_NOALLOC_MAX_POINTS 32768
+#define SSASDPD_NOALLOC_MAX_QUERIES 32768
+#define SSASDPD_NOALLOC_MAX_WIDTH 32
+int ssasdpd_fr_search_noalloc(const double *pts, int32_t n, const double *q, int32_t m,
+                              int32_t dim, double h, int64_t *row_ptr, int32_t *idx,
+                              double *r2);
+
 /* Event trace of the last run / advance when SSASDPD_FLAG_TRACE is set:
  * per event (time, voxel, kind 0=reaction 1=diffusion, reaction or species,
  * destination or -1). */

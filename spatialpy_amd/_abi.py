@@ -12,6 +12,11 @@ ABI_VERSION = 3
 
 OK = 0
 TIMEOUT = 8
+CAPACITY = 10
+# limits of ssasdpd_fr_search_noalloc (include/ssasdpd.h)
+NOALLOC_MAX_POINTS = 32768
+NOALLOC_MAX_QUERIES = 32768
+NOALLOC_MAX_WIDTH = 32
 STATUS_NAMES = {
     0: "OK", 1: "BAD_DESC", 2: "NAN_STATE", 3: "NEGATIVE_STATE",
     4: "PROPENSITY_OVERFLOW", 5: "DIFFUSION_OVERFLOW", 6: "NAN_DIJ", 7: "HIP",
@@ -130,6 +135,8 @@ def _declare(lib):
     lib.ssasdpd_fr_search.argtypes = [_dp, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_double,
                                       _i64p, _ip, _dp]
     lib.ssasdpd_set_trace_capacity.argtypes = [P, C.c_uint64]
+    lib.ssasdpd_fr_search_noalloc.argtypes = [_dp, C.c_int32, _dp, C.c_int32, C.c_int32, C.c_double,
+                                              _i64p, _ip, _dp]
     lib.ssasdpd_set_timeout.argtypes = [P, C.c_double]
     lib.ssasdpd_get_trace.argtypes = [P, C.POINTER(Event), C.c_uint64, _u64p]
     lib.ssasdpd_mt19937_64_stream.argtypes = [C.c_uint64, C.c_uint64, _u64p, _dp]
@@ -175,7 +182,7 @@ def _declare(lib):
                  "ssasdpd_mpn_set_params", "ssasdpd_mpn_set_scales", "ssasdpd_mpn_add_sample",
                  "ssasdpd_mpn_add_engine_sample", "ssasdpd_mpn_clear_samples", "ssasdpd_mpn_forward",
                  "ssasdpd_mpn_grad", "ssasdpd_mpn_train", "ssasdpd_set_surrogate", "ssasdpd_surrogate_stats",
-                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward", "ssasdpd_vm_eval", "ssasdpd_set_timeout"]:
+                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward", "ssasdpd_vm_eval", "ssasdpd_set_timeout", "ssasdpd_fr_search_noalloc"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

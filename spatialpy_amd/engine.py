@@ -236,6 +236,26 @@ def fr_search(pts, q, dim, h):
     return rp, idx, r2
 
 
+def fr_search_noalloc(pts, q, dim, h):
+    """fr_search through ssasdpd_fr_search_noalloc: the same lists, with
+    every device buffer in the library's fixed scratch array (no device
+    allocation; at most NOALLOC_MAX_POINTS points and NOALLOC_MAX_QUERIES
+    queries of at most NOALLOC_MAX_WIDTH neighbours)."""
+    lib = _abi.load_library()
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    m = q.shape[0]
+    rp = np.zeros(m + 1, np.int64)
+    _abi.check(lib, lib.ssasdpd_fr_search_noalloc(_p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), m, dim, h,
+                                                  _p(rp, C.c_int64), None, None), "fr_search_noalloc")
+    idx = np.zeros(rp[-1], np.int32)
+    r2 = np.zeros(rp[-1], np.float64)
+    _abi.check(lib, lib.ssasdpd_fr_search_noalloc(_p(pts, C.c_double), pts.shape[0], _p(q, C.c_double), m, dim, h,
+                                                  _p(rp, C.c_int64), _p(idx, C.c_int32), _p(r2, C.c_double)),
+               "fr_search_noalloc")
+    return rp, idx, r2
+
+
 def vm_eval(code, consts, start, x, vol, t, df=None, x_is_double=False):
     """Evaluate one compiled propensity program (spatialpy_amd.expression)
     on the GPU with the engine's interpreter for m argument sets: x [m][n_x]
