@@ -166,15 +166,39 @@ class TimeSpan:
 
 
 class PlaceInitialCondition:
-    """initialcondition.py:35-75: count molecules at the vertex closest to location."""
+    """initialcondition.py:35-75: count molecules at the vertex that
+    Domain.closest_vertex returns for location (core/domain.py:460-477):
+    the first vertex of smallest positive distance -- a vertex at distance
+    exactly 0 is skipped, so a location on a vertex places the molecules on
+    its nearest other vertex."""
 
     def __init__(self, species, count, location=None):
         self.species, self.count, self.location = species, count, location if location is not None else [0, 0, 0]
 
     def apply(self, model):
         s = model._species_index(self.species)
-        vtx = int(np.argmin(np.sum((model.domain.vertices - np.asarray(self.location)) ** 2, axis=1)))
+        vtx = closest_vertex(model.domain.vertices, self.location)
         model.u0[s, vtx] += self.count
+
+
+def closest_vertex(vertices, point):
+    """core/domain.py:460-477: the first index i with the smallest positive
+    numpy.linalg.norm(vertices[i] - point).  Candidates are found with
+    vectorised squared distances; the reference's own norm decides among
+    those within rounding of the minimum, in index order."""
+    v = np.asarray(vertices, dtype=float)
+    p = np.asarray(point, dtype=float)
+    d2 = np.einsum("ij,ij->i", v - p, v - p)
+    pos = d2 > 0
+    if not pos.any():
+        return None
+    m = d2[pos].min()
+    best, best_d = None, None
+    for i in np.nonzero(pos & (d2 <= m * (1 + 1e-9)))[0]:
+        d = np.linalg.norm(v[i, :] - p)
+        if d > 0 and (best_d is None or d < best_d):
+            best, best_d = int(i), d
+    return best
 
 
 class UniformInitialCondition:

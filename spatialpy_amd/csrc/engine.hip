@@ -131,6 +131,9 @@ struct ssasdpd_engine {
     EdgeRec *erow_alt = nullptr;
     // binning / tree
     double *xtree = nullptr, *spts = nullptr, *d_bbox = nullptr;
+    PRec *rec = nullptr;        // packed gather records, written by the predictor
+    bool any_solid = true;      // some particle is fixed (solidTag, static)
+    bool rec_ready = false;     // rec holds this step's post-predictor state
     // pinned landing slot for the per-step bbox readback: a pageable copy goes
     // through the runtime's shared staging buffer, which can hold the particle
     // stream behind a running event loop
@@ -231,7 +234,7 @@ struct ssasdpd_engine {
         la_done = nullptr;
         if (ps) (void)hipStreamDestroy(ps);
         ps = nullptr;
-        dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp);
+        dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp); dfree(rec);
         if (h_bbox) (void)hipHostFree(h_bbox);
         h_bbox = nullptr;
         dfree(sorted); dfree(cell_of); dfree(cell_count); dfree(cell_start); dfree(cell_fill);
@@ -398,7 +401,9 @@ struct ssasdpd_engine {
             }
         size_t n3 = 3 * (size_t)n;
         x = dalloc<double>(n3); v = dalloc<double>(n3); vt = dalloc<double>(n3); F = dalloc<double>(n3);
-        Fbp = dalloc<double>(n3); normal = dalloc<double>(n3); xtree = dalloc<double>(n3); spts = dalloc<double>(n3);
+        Fbp = dalloc<double>(n3); normal = dalloc<double>(n3); xtree = dalloc<double>(n3);
+        spts = dalloc<double>(4 * (size_t)n);
+        rec = dalloc<PRec>(n);
         rho = dalloc<double>(n); old_rho = dalloc<double>(n); Frho = dalloc<double>(n); mass = dalloc<double>(n);
         nu = dalloc<double>(n); bvf = dalloc<double>(n);
         type = dalloc<int>(n); solid = dalloc<int>(n);
@@ -423,6 +428,7 @@ struct ssasdpd_engine {
         std::vector<int> ty(d->type, d->type + n), so(d->fixed, d->fixed + n);
         h2d(st, type, ty.data(), n);
         h2d(st, solid, so.data(), n);
+        any_solid = std::any_of(so.begin(), so.end(), [](int v) { return v != 0; });
         h2d(st, mass, d->mass, n);
         x0 = to_soa3(d->x, n);
         v0 = to_soa3(d->v, n);
@@ -607,15 +613,18 @@ struct ssasdpd_engine {
         for (int attempt = 0; attempt < 4; attempt++) {
             HIP_OK(hipMemsetAsync(&derr->overflow, 0, sizeof(int), st));
             HIP_OK(hipMemsetAsync(&derr->max_deg, 0, sizeof(int), st));
-            // queries in particle order: each wave's list writes are coalesced
+            // queries in particle order: each wave's list writes are
+            // coalesced (cell order shares candidate cells but scatters the
+            // writes, and measured slower); each sorted list is finished (r,
+            // D_ij with the current rho, rows) in the same kernel
+            const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr};
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
-                      deg, derr, nullptr);
+                      deg, derr, nullptr, &fa);
             DevErr e = read_err();
             max_deg_seen = std::max(max_deg_seen, e.max_deg);
             if (!e.overflow) break;
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
-        finish_lists(st, n, W, deg, nidx, nr2, nr, Dij, mass, rho, h, derr, type, ntype, erow);
         ph_end(1);
         lists_built = true;
     }
@@ -628,6 +637,8 @@ struct ssasdpd_engine {
         a.x = x; a.v = v; a.vt = vt; a.F = F; a.Fbp = Fbp; a.rho = rho; a.old_rho = old_rho; a.Frho = Frho;
         a.bvf = bvf; a.normal = normal; a.mass = mass; a.nu = nu; a.type = type; a.solid = solid; a.C = C; a.Q = Q;
         a.Dmat = Dmat; a.Ndense = Ndense; a.rx = rx; a.bc = bc; a.deg = deg; a.nidx = nidx; a.nr = nr; a.err = derr;
+        a.rec = rec;
+        a.no_solids = any_solid ? 0 : 1;
         a.fp32 = (flags & SSASDPD_FLAG_FP32_FORCES) ? 1 : 0;
         a.vm_code = vm_code; a.vm_k = vm_k; a.data_fn = dfn; a.n_df = n_df;
         return a;
@@ -693,6 +704,7 @@ struct ssasdpd_engine {
     // trajectories shares it.
     void particle_phase(unsigned step) {
         const bool moving = static_domain == 0;
+        rec_ready = false;
         if (step == 0 || moving) {
             bin_tree_points();
             // substep 0 lists (simulate.cpp:61-63).  On a moving domain the
@@ -703,8 +715,9 @@ struct ssasdpd_engine {
         }
         SdpdArgs sa = sdpd_args(step);
         ph_begin(2);
-        sdpd_predict(st, sa);                      // substep 0
+        sdpd_predict(st, sa);                      // substep 0 (+ the gather records)
         ph_end(2);
+        rec_ready = true;
         if (step > 0 && moving) {
             search();                              // substep 1 lists (simulate.cpp:121-123)
             sa = sdpd_args(step);
@@ -1507,6 +1520,7 @@ int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp,
     a.Sc = dQ ? a.Sc : 0;
     a.Rc = dQ ? a.Rc : 0;
     a.fp32 = fp32 ? 1 : 0;
+    sdpd_pack(e->st, a);      // gather records of the current state
     sdpd_forces(e->st, a);
     std::vector<double> tmp(n3);
     if (F) { d2h(e->st, tmp.data(), dF, n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, F, (int)n); }
@@ -1993,10 +2007,11 @@ int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx, dou
     row_ptr[n] = off;
     if (!idx) return 0;
     std::vector<int> ni((size_t)W * n);
-    std::vector<double> nr((size_t)W * n), nd((size_t)W * n);
+    std::vector<double> nr((size_t)W * n);
+    std::vector<EdgeRec> er((size_t)W * n);
     d2h(e->st, ni.data(), e->nidx, (size_t)W * n);
     d2h(e->st, nr.data(), e->nr, (size_t)W * n);
-    d2h(e->st, nd.data(), e->Dij, (size_t)W * n);
+    d2h(e->st, er.data(), e->erow, (size_t)W * n);
     HIP_OK(hipStreamSynchronize(e->st));
     const double h = e->h, a = e->alpha;
     for (int i = 0; i < n; i++)
@@ -2010,7 +2025,7 @@ int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx, dou
                 double R = r / h;
                 dWdr[o] = a * (-12 * r / (h * h)) * ((1 - R) * (1 - R));   // particle.cpp:178
             }
-            if (Dij) Dij[o] = nd[s];
+            if (Dij) Dij[o] = er[(size_t)i * W + k].D;
         }
     return 0;
     GUARD_END
@@ -2024,7 +2039,7 @@ int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m, 
     HIP_OK(hipStreamCreate(&st));
     std::vector<double> hp = to_soa3(pts, n), hq = to_soa3(q, m);
     double *dp = dalloc<double>(3 * (size_t)n), *dq = dalloc<double>(3 * (size_t)std::max(m, 1));
-    double *sp = dalloc<double>(3 * (size_t)n), *bb = dalloc<double>(8);
+    double *sp = dalloc<double>(4 * (size_t)n), *bb = dalloc<double>(8);
     unsigned long long *bbt = dalloc<unsigned long long>(8);
     int *srt = dalloc<int>(n), *cof = dalloc<int>(n);
     h2d(st, dp, hp.data(), 3 * (size_t)n);

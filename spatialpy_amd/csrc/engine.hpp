@@ -9,6 +9,7 @@
 //         Ddiag FP64 [n][S], firing times FP64 [n] + a 64-ary min tree
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -104,6 +105,19 @@ struct Grid {
     long long ncell;
 };
 
+// What the neighbour loops gather from particle j, packed after the
+// predictor: one 128-byte line per neighbour instead of a load from each of
+// twelve component arrays.  Written by k_predict (and k_pack); the force,
+// D_ij, density-filter and boundary-volume-fraction kernels read it.
+struct __align__(16) PRec {
+    double x[3], v[3], vt[3];
+    double nu, rho, mass;        // rho, mass: one aligned 16-byte pair
+    int type, solid, pad0, pad1;
+    double pad2[2];
+};
+static_assert(sizeof(PRec) == 128, "PRec is one 128-byte line");
+static_assert(offsetof(PRec, rho) % 16 == 0, "rho, mass load as one double2");
+
 struct Bufs;   // defined in engine.hip
 
 // ---- launch wrappers (implemented in nbr.hip / sdpd.hip / ssa.hip) -------
@@ -114,17 +128,9 @@ struct Bufs;   // defined in engine.hip
 void grid_setup(const double *pts_host_bbox, int n, int dim, double h, Grid &g);
 void bin_points(hipStream_t s, const double *pts, int n, int dim, const Grid &g, int *cell_of,
                 int *cell_count, int *cell_start, int *cell_fill, int *sorted, double *spts,
-                void *scan_tmp);
+                void *scan_tmp);   // spts: sorted points [n][4] (x, y, z, 0)
 void compute_bbox(hipStream_t s, const double *pts, int n, int dim, double *d_bbox /*[6]*/,
                   unsigned long long *tmp);
-void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted,
-               const int *cell_start, int n, int dim, double h, const Grid &g, const KdDev *kd,
-               int W, int *out_idx, double *out_r2, int *deg, DevErr *err, const int *query_order);
-void kd_build(hipStream_t s, const double *pts, int n, int dim, KdDev &kd, void *work, size_t work_bytes);
-size_t kd_work_bytes(int n);
-size_t scan_tmp_bytes(long long n);
-void exclusive_scan_i32(hipStream_t s, const int *in, int *out, long long n, void *tmp);
-
 // One edge of a voxel's neighbour row as the event loop reads it: a row is
 // contiguous (erow[i*W + k]), so one event's row load touches a few cache
 // lines instead of one per slot of the neighbour-major ELL arrays.
@@ -134,17 +140,34 @@ struct EdgeRec {
     double D;     // D_ij
 };
 
-// D_ij and r from r2 (add_to_neighbor_list, ENG/src/particle.cpp:150-210),
-// plus the voxel-major rows erow.
-void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2,
-                  double *r, double *Dij, const double *mass, const double *rho, double h,
-                  DevErr *err, const int *type, int *ntype, EdgeRec *erow);
+// The per-edge finish of add_to_neighbor_list (r, D_ij; ENG/src/particle.cpp:
+// 150-210) plus the neighbour's type and the event loop's voxel-major row,
+// done by the search itself once a query's list is sorted.  Own mass and rho
+// from the arrays; the neighbour's from rec when non-null, else the arrays.
+struct FinishArgs {
+    double *r, *Dij;
+    int *ntype;
+    EdgeRec *erow;
+    const double *mass, *rho;
+    const int *type;
+    const PRec *rec;
+};
+void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted,
+               const int *cell_start, int n, int dim, double h, const Grid &g, const KdDev *kd,
+               int W, int *out_idx, double *out_r2, int *deg, DevErr *err, const int *query_order,
+               const FinishArgs *fin = nullptr);
+void kd_build(hipStream_t s, const double *pts, int n, int dim, KdDev &kd, void *work, size_t work_bytes);
+size_t kd_work_bytes(int n);
+size_t scan_tmp_bytes(long long n);
+void exclusive_scan_i32(hipStream_t s, const int *in, int *out, long long n, void *tmp);
+
 
 // SDPD substeps.
 struct SdpdArgs {
     int n, dim, Sc, Rc, T, static_domain, nbc, W;
     int fp32;                // pair-force terms in FP32 (FP64 accumulation)
     int skip_F;              // leave F at gravity (the surrogate supplies it)
+    int no_solids;           // no particle is solid (the BVF then needs no neighbour data)
     unsigned step;
     double dt, h, rho0, c0, P0, alpha;
     double g[3];
@@ -159,6 +182,7 @@ struct SdpdArgs {
     const BcRule *bc;
     const int *deg, *nidx;
     const double *nr;
+    PRec *rec;               // [n] packed gather records (after the predictor)
     DevErr *err;
     // expression-VM programs (kind-4 reactions) and data functions [n][n_df]
     const uint32_t *vm_code;
@@ -170,6 +194,8 @@ void sdpd_predict(hipStream_t s, const SdpdArgs &a);
 void sdpd_forces(hipStream_t s, const SdpdArgs &a);
 void sdpd_correct(hipStream_t s, const SdpdArgs &a);
 void sdpd_bvf(hipStream_t s, const SdpdArgs &a);
+// rec from the current particle arrays (for force evaluations outside a step)
+void sdpd_pack(hipStream_t s, const SdpdArgs &a);
 
 // RDME.
 struct RdmeArgs {

@@ -223,7 +223,10 @@ __global__ void k_cell_scatter(const double *pts, int n, int dim, const int *cel
     int c = cell_of[i];
     int p = start[c] + atomicAdd(&fill[c], 1);
     sorted[p] = i;
-    for (int d = 0; d < 3; d++) spts[(size_t)d * n + p] = d < dim ? pts[(size_t)d * n + i] : 0.0;
+    // one 32-byte record per point: a candidate is two 16-byte loads
+    double2 *o = (double2 *)(spts + 4 * (size_t)p);
+    o[0] = make_double2(pts[i], dim > 1 ? pts[(size_t)n + i] : 0.0);
+    o[1] = make_double2(dim > 2 ? pts[2 * (size_t)n + i] : 0.0, 0.0);
 }
 
 void bin_points(hipStream_t s, const double *pts, int n, int dim, const Grid &g, int *cell_of, int *cell_count,
@@ -638,11 +641,22 @@ __device__ __forceinline__ void sort_list(IA I, DA D, int cnt, const KdView &kd,
     }
 }
 
+// D_ij of one edge (add_to_neighbor_list, particle.cpp:160-187): the 3-D
+// constant of wfd in every dimension, operation for operation.
+__device__ __forceinline__ double edge_dij(double mi, double ri, double mj, double rj, double rr2, double rr,
+                                           double h) {
+    double ih = 1.0 / h;
+    double ihsq = ih * ih;
+    double dhr = h - rr;
+    double wfd = -25.066903536973515383e0 * dhr * dhr * ihsq * ihsq * ihsq * ih;
+    return -2.0 * (mi * mj) / (mi + mj) * (ri + rj) / (ri * rj) * rr2 * wfd / (rr2 + 0.01 * h * h);
+}
+
 __global__ void __launch_bounds__(WAVE)
 k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts, const int *__restrict__ sorted,
             const int *__restrict__ cell_start, int n, int dim, double h, Grid g, KdView kd, int W,
             int *__restrict__ out_idx, double *__restrict__ out_r2, int *__restrict__ deg, DevErr *err,
-            const int *__restrict__ order) {
+            const int *__restrict__ order, FinishArgs fin, int has_fin) {
     __shared__ int l_idx[FR_SLOTS * WAVE];
     __shared__ double l_r2[FR_SLOTS * WAVE];
     const int tl = (int)threadIdx.x;
@@ -673,11 +687,13 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
             int row = (cz * g.dims[1] + cy) * g.dims[0];
             int pb = cell_start[row + lo[0]], pe = cell_start[row + hi[0] + 1];
             for (int p = pb; p < pe; p++) {
+                const double2 *sp = (const double2 *)(spts + 4 * (size_t)p);
+                const double2 pxy = sp[0];
                 double dist = 0.0;
-                double t0 = qp[0] - spts[p];
+                double t0 = qp[0] - pxy.x;
                 dist = dist + t0 * t0;
-                if (dim > 1) { double t1 = qp[1] - spts[(size_t)n + p]; dist = dist + t1 * t1; }
-                if (dim > 2) { double t2 = qp[2] - spts[2 * (size_t)n + p]; dist = dist + t2 * t2; }
+                if (dim > 1) { double t1 = qp[1] - pxy.y; dist = dist + t1 * t1; }
+                if (dim > 2) { double t2 = qp[2] - sp[1].x; dist = dist + t2 * t2; }
                 if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) {
                     const int j = sorted[p];
                     if (cnt < L) {
@@ -699,9 +715,45 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
         deg[qi] = cnt;
         if (cnt <= L) {
             sort_list(LI, LD, cnt, kd, qp);
-            for (int k = 0; k < cnt; k++) { GI(k) = LI(k); GD(k) = LD(k); }
+            // with the finish, r2 is only the spill area of long lists
+            for (int k = 0; k < cnt; k++) {
+                GI(k) = LI(k);
+                if (!has_fin) GD(k) = LD(k);
+            }
         } else {
             sort_list(GI, GD, cnt, kd, qp);
+        }
+        if (has_fin) {
+            // r, D_ij, the neighbour's type and the voxel-major row, in list order
+            const double mi = fin.mass[qi], ri = fin.rho[qi];
+            for (int k = 0; k < cnt; k++) {
+                const int j = cnt <= L ? LI(k) : GI(k);
+                const double rr2 = cnt <= L ? LD(k) : GD(k);
+                const double rr = sqrt(rr2);
+                double mj, rj;
+                int tj;
+                if (fin.rec) {
+                    const PRec *pj = fin.rec + j;
+                    const double2 rm = *(const double2 *)&pj->rho;
+                    rj = rm.x;
+                    mj = rm.y;
+                    tj = pj->type;
+                } else {
+                    mj = fin.mass[j];
+                    rj = fin.rho[j];
+                    tj = fin.type[j];
+                }
+                const double D = edge_dij(mi, ri, mj, rj, rr2, rr, h);
+                if (D != D) {
+                    atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
+                    atomicMin(&err->particle, qi);
+                }
+                const size_t e = (size_t)k * m + qi;
+                fin.r[e] = rr;
+                fin.Dij[e] = D;
+                fin.ntype[e] = tj;
+                fin.erow[(size_t)qi * W + k] = EdgeRec{j, tj, D};
+            }
         }
     }
     atomicMax(&err->max_deg, cnt);
@@ -709,55 +761,15 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
 
 void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted, const int *cell_start,
                int n, int dim, double h, const Grid &g, const KdDev *kd, int W, int *out_idx, double *out_r2,
-               int *deg, DevErr *err, const int *order) {
+               int *deg, DevErr *err, const int *order, const FinishArgs *fin) {
     KdView v{};
     if (kd && kd->valid) {
         v.parent = kd->parent; v.depth = kd->depth; v.cd = kd->cd; v.is_hi = kd->is_hi;
         v.leaf_of = kd->leaf_of; v.cv = kd->cv; v.valid = 1;
     }
+    const FinishArgs f = fin ? *fin : FinishArgs{};
     hipLaunchKernelGGL(k_fr_search, xcd_grid((m + WAVE - 1) / WAVE), dim3(WAVE), 0, s, q, m, spts, sorted, cell_start,
-                       n, dim, h, g, v, W, out_idx, out_r2, deg, err, order);
-}
-
-// ---------------------------------------------------------------------------
-// r = sqrt(r2) and D_ij per edge (add_to_neighbor_list, particle.cpp:160-207).
-__global__ void k_finish(int n, int W, const int *deg, const int *idx, const double *r2, double *r, double *Dij,
-                         const double *mass, const double *rho, double h, DevErr *err, const int *type,
-                         int *ntype, EdgeRec *erow) {
-    const int chunk = xcd_chunk((n + 255) / 256);
-    if (chunk < 0) return;
-    int i = chunk * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int dg = deg[i];
-    double mi = mass[i], ri = rho[i];
-    for (int k = 0; k < dg; k++) {
-        size_t e = (size_t)k * n + i;
-        int j = idx[e];
-        double rr2 = r2[e];
-        double rr = sqrt(rr2);
-        double ih = 1.0 / h;
-        double ihsq = ih * ih;
-        double dhr = h - rr;
-        double wfd = -25.066903536973515383e0 * dhr * dhr * ihsq * ihsq * ihsq * ih;
-        double mj = mass[j], rj = rho[j];
-        double D = -2.0 * (mi * mj) / (mi + mj) * (ri + rj) / (ri * rj) * rr2 * wfd / (rr2 + 0.01 * h * h);
-        if (D != D) {
-            atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
-            atomicMin(&err->particle, i);
-        }
-        const int tj = type[j];
-        r[e] = rr;
-        Dij[e] = D;
-        ntype[e] = tj;
-        erow[(size_t)i * W + k] = EdgeRec{j, tj, D};
-    }
-}
-
-void finish_lists(hipStream_t s, int n, int W, const int *deg, const int *idx, const double *r2, double *r,
-                  double *Dij, const double *mass, const double *rho, double h, DevErr *err, const int *type,
-                  int *ntype, EdgeRec *erow) {
-    hipLaunchKernelGGL(k_finish, xcd_grid((n + 255) / 256), dim3(256), 0, s, n, W, deg, idx, r2, r, Dij, mass, rho, h,
-                       err, type, ntype, erow);
+                       n, dim, h, g, v, W, out_idx, out_r2, deg, err, order, f, fin ? 1 : 0);
 }
 
 }  // namespace ssa

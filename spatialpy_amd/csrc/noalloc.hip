@@ -32,8 +32,8 @@ constexpr size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t CAP_P = (size_t)NA_MAX_POINTS + 1, CAP_Q = (size_t)NA_MAX_QUERIES + 1;
 constexpr size_t O_STAGE = 0;                                                 // AoS upload [3][max(n, m)]
 constexpr size_t O_PTS = O_STAGE + al(sizeof(double) * 3 * std::max(CAP_P, CAP_Q));
-constexpr size_t O_SPTS = O_PTS + al(sizeof(double) * 3 * CAP_P);           // sorted tree points
-constexpr size_t O_Q = O_SPTS + al(sizeof(double) * 3 * CAP_P);
+constexpr size_t O_SPTS = O_PTS + al(sizeof(double) * 3 * CAP_P);           // sorted tree points [n][4]
+constexpr size_t O_Q = O_SPTS + al(sizeof(double) * 4 * CAP_P);
 constexpr size_t O_SORTED = O_Q + al(sizeof(double) * 3 * CAP_Q);
 constexpr size_t O_CELL_OF = O_SORTED + al(sizeof(int) * CAP_P);
 constexpr size_t O_CC = O_CELL_OF + al(sizeof(int) * CAP_P);

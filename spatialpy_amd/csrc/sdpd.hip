@@ -46,6 +46,41 @@ __device__ __forceinline__ void apply_bcs(const SdpdArgs &a, int i) {
 
 __device__ __forceinline__ bool bad(double x) { return isnan(x) || !isfinite(x); }
 
+// Particle i's gather record from the arrays (eight 16-byte stores).
+__device__ __forceinline__ void store_rec(const SdpdArgs &a, int i) {
+    const size_t n = a.n;
+    double2 *p = (double2 *)(a.rec + i);
+    p[0] = make_double2(a.x[i], a.x[n + i]);
+    p[1] = make_double2(a.x[2 * n + i], a.v[i]);
+    p[2] = make_double2(a.v[n + i], a.v[2 * n + i]);
+    p[3] = make_double2(a.vt[i], a.vt[n + i]);
+    p[4] = make_double2(a.vt[2 * n + i], a.nu[i]);
+    p[5] = make_double2(a.rho[i], a.mass[i]);
+    ((int4 *)p)[6] = make_int4(a.type[i], a.solid[i], 0, 0);
+    p[7] = make_double2(0.0, 0.0);
+}
+
+// Neighbour j's record: x, v, vt, nu, rho, mass (six 16-byte loads of one
+// 128-byte line).
+__device__ __forceinline__ void load_rec(const PRec *r, double x[3], double v[3], double vt[3], double &nu,
+                                         double &rho, double &mass) {
+    const double2 *p = (const double2 *)r;
+    const double2 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5];
+    x[0] = a0.x; x[1] = a0.y; x[2] = a1.x;
+    v[0] = a1.y; v[1] = a2.x; v[2] = a2.y;
+    vt[0] = a3.x; vt[1] = a3.y; vt[2] = a4.x;
+    nu = a4.y; rho = a5.x; mass = a5.y;
+}
+
+__device__ __forceinline__ double2 load_rho_mass(const PRec *r) { return *(const double2 *)&r->rho; }
+
+__global__ void k_pack(SdpdArgs a) {
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    int i = chunk * blockDim.x + threadIdx.x;
+    if (i < a.n) store_rec(a, i);
+}
+
 __global__ void k_predict(SdpdArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
     if (chunk < 0) return;
@@ -78,6 +113,7 @@ __global__ void k_predict(SdpdArgs a) {
     a.Frho[i] = 0.0;
     for (int s = 0; s < a.Sc; s++) a.Q[(size_t)i * a.Sc + s] = 0.0;
     a.old_rho[i] = a.rho[i];
+    store_rec(a, i);
 }
 
 // generated det<name> functions (solver.py:160-190), called with
@@ -146,15 +182,21 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
         const Real r = (Real)rd;
         const Real R = r / hR;
         const Real dWdr = alpha * ((Real)-12 * r / (hR * hR)) * (((Real)1 - R) * ((Real)1 - R));
-        const Real rhoj = (Real)a.rho[j], mj = (Real)a.mass[j];
+        double xj[3], vj[3], vtj[3], nujd, rhojd, mjd;
+        if (momentum) {
+            load_rec(a.rec + j, xj, vj, vtj, nujd, rhojd, mjd);
+        } else {
+            const double2 rm = load_rho_mass(a.rec + j);
+            rhojd = rm.x;
+            mjd = rm.y;
+        }
+        const Real rhoj = (Real)rhojd, mj = (Real)mjd;
         const Real inv_rh = (Real)1.0 / (r + (Real)0.001 * hR);
         if (momentum) {
-            const double vj[3] = {a.v[j], a.v[n + j], a.v[2 * n + j]};
-            const double vtj[3] = {a.vt[j], a.vt[n + j], a.vt[2 * n + j]};
-            const Real nuj = (Real)a.nu[j];
+            const Real nuj = (Real)nujd;
             Real dv_dx = 0;
             for (int d = 0; d < a.dim; d++) {
-                dx[d] = (Real)(xi[d] - a.x[d * n + j]);
+                dx[d] = (Real)(xi[d] - xj[d]);
                 dv[d] = (Real)(vi[d] - vj[d]);
                 dv_dx += dv[d] * dx[d];
             }
@@ -194,7 +236,7 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
         if (Sc > 0) {
             const double Rd = rd / h;
             const double dWdrd = a.alpha * (-12 * rd / (h * h)) * ((1 - Rd) * (1 - Rd));
-            const double rhoid = a.rho[i], rhojd = a.rho[j], mid = a.mass[i], mjd = a.mass[j];
+            const double rhoid = a.rho[i], mid = a.mass[i];
             const double wfd = (1.0 / (rd + 0.001 * h)) * dWdrd;
             const double dQc_base = 2.0 * ((mid * mjd) / (mid + mjd)) * ((rhoid + rhojd) / (rhoid * rhojd)) *
                                     (rd * rd) * wfd / ((rd * rd) + 0.01 * h * h);
@@ -242,7 +284,7 @@ __global__ void k_correct(SdpdArgs a) {
             if (R > 1.0) continue;
             if (r == 0.0) continue;
             const double W = a.alpha * ((1 + 3 * R) * cube(1 - R));
-            num += a.old_rho[a.nidx[e]] * W;
+            num += a.rec[a.nidx[e]].rho * W;     // old_rho of j (the record is written with it)
             den += W;
         }
         a.rho[i] = num / den;
@@ -268,6 +310,23 @@ __global__ void k_bvf(SdpdArgs a) {
         for (int d = 0; d < a.dim; d++) { a.normal[d * n + i] = 0.0; a.vt[d * n + i] = 0.0; }
         const double xi[3] = {a.x[i], a.x[n + i], a.x[2 * n + i]};
         const int dg = a.deg[i];
+        if (a.no_solids) {
+            // no solid neighbour: vos and nw stay 0, so phi = |0 / vtot| is 0
+            // when some neighbour has W > 0 (0 < r < h, every (m/rho)^2 > 0)
+            // and 0/0 = NaN otherwise; the normal is -0/0 = NaN and the
+            // velocity is never reflected (phi >= 0.5 is false for both)
+            bool pos = false;
+            for (int k = 0; k < dg && !pos; k++) {
+                const double r = a.nr[(size_t)k * n + i], R = r / h;
+                pos = !(R > 1.0) && r != 0.0 && (1 + 3 * R) * cube(1 - R) > 0.0;
+            }
+            const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+            for (int d = 0; d < 3; d++) a.normal[d * n + i] = -qnan;
+            a.bvf[i] = pos ? 0.0 : qnan;
+            for (int s = 0; s < a.Sc; s++) a.C[(size_t)i * a.Sc + s] += a.Q[(size_t)i * a.Sc + s] * a.dt * 0.5;
+            apply_bcs(a, i);
+            return;
+        }
         for (int k = 0; k < dg; k++) {
             const size_t e = (size_t)k * n + i;
             const int j = a.nidx[e];
@@ -276,10 +335,15 @@ __global__ void k_bvf(SdpdArgs a) {
             if (r == 0.0) continue;
             const double W = a.alpha * ((1 + 3 * R) * cube(1 - R));
             const double dWdr = a.alpha * (-12 * r / (h * h)) * ((1 - R) * (1 - R));
-            for (int d = 0; d < a.dim; d++) dx[d] = xi[d] - a.x[d * n + j];
-            const double mr = a.mass[j] / a.rho[j];
+            // x, mass and solid of j from its record (unchanged since the
+            // predictor); rho[j] after the corrector from the array
+            const PRec *pj = a.rec + j;
+            const double2 x01 = *(const double2 *)&pj->x[0];
+            const double xj[3] = {x01.x, x01.y, pj->x[2]};
+            for (int d = 0; d < a.dim; d++) dx[d] = xi[d] - xj[d];
+            const double mr = pj->mass / a.rho[j];
             const double mv = mr * mr;
-            const bool sj = a.solid[j] != 0;
+            const bool sj = pj->solid != 0;
             if (sj) vos += mv * W;
             vtot += mv * W;
             if (sj)
@@ -337,5 +401,6 @@ void sdpd_correct(hipStream_t s, const SdpdArgs &a) {
     if (!a.static_domain) hipLaunchKernelGGL(k_correct, grid1(a.n), dim3(256), 0, s, a);
 }
 void sdpd_bvf(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_bvf, grid1(a.n), dim3(256), 0, s, a); }
+void sdpd_pack(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_pack, grid1(a.n), dim3(256), 0, s, a); }
 
 }  // namespace ssa

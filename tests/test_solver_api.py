@@ -141,3 +141,19 @@ def test_result_export_vtk_matches_oracle_files(tmp_path, oracle):
     assert len(files) == 12
     for f in files:
         assert (tmp_path / "gpu" / f).read_text() == (tmp_path / "cpu" / f).read_text(), f
+
+
+def test_place_initial_condition_follows_closest_vertex():
+    """Domain.closest_vertex (core/domain.py:460-477) skips a vertex at
+    distance exactly 0: a PlaceInitialCondition at a vertex's coordinates
+    lands on its nearest other vertex (ties: the lower index)."""
+    from spatialpy_amd.core.model import closest_vertex
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.4, 0.0, 0.0]])
+    assert closest_vertex(x, [0.0, 0.0, 0.0]) == 3
+    assert closest_vertex(x, [0.1, 0.0, 0.0]) == 0
+    assert closest_vertex(x, [0.4, 0.0, 0.0]) == 0       # itself skipped; 0 at 0.4 beats 1 at 0.6
+    assert closest_vertex(x[:3], [0.0, 0.0, 0.0]) == 1   # 1 and 2 tie at distance 1
+    model = birth_death_model()
+    model.listOfInitialConditions = [sp.PlaceInitialCondition("Rabbits", 5, model.domain.vertices[0])]
+    model.compile_prep()
+    assert model.u0[0].sum() == 5 and model.u0[0, 0] == 0
