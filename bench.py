@@ -535,6 +535,93 @@ def bench_cylinder(args):
     print(json.dumps(out))
 
 
+def _short_kernel(name):
+    import re
+    return re.sub(r"\(.*", "", name).replace("void ", "").strip()
+
+
+def profile_summary(dbs, steps, calibration_mib):
+    """Summarise rocprofv3 result databases (rocpd SQLite) of this script.
+
+    A database of a cylinder run covers every dispatch of the run; the
+    window is its last `steps` event-loop steps (from the end of the
+    (steps+1)-th last k_ssa launch to the end of the last one), i.e. the
+    timed window of `bench.py --no-full-run`.  Per kernel: dispatches in the
+    window, mean duration and, for counter databases, the mean value of each
+    counter per dispatch.  A database without k_ssa launches is taken as the
+    copy calibration (`--copy-calibration MIB`): its copy kernel's FETCH_SIZE
+    and WRITE_SIZE per KiB copied give the factors that turn those counters
+    into HBM bytes, and the HBM traffic per step and per k_ssa launch is
+    printed with them."""
+    import collections
+    import sqlite3
+    tables, scale = [], {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        views = {r[0] for r in c.execute("select name from sqlite_master where type in ('view', 'table')")}
+        rows = []
+        if "kernels" in views:
+            rows = c.execute("select dispatch_id, name, start, end, duration from kernels").fetchall()
+        if not rows and "counters_collection" in views:
+            rows = c.execute("select dispatch_id, min(kernel_name), min(start), min(end), min(duration) "
+                             "from counters_collection group by dispatch_id").fetchall()
+        rows.sort(key=lambda r: r[2])
+        counters = collections.defaultdict(dict)
+        if "counters_collection" in views:
+            for d, name, val in c.execute("select dispatch_id, counter_name, sum(value) from counters_collection "
+                                          "group by dispatch_id, counter_name"):
+                counters[d][name] = float(val)
+        ssa = [r for r in rows if _short_kernel(r[1]).startswith("ssa::k_ssa")]
+        calib = not ssa
+        if calib:
+            win = rows
+        else:
+            if len(ssa) <= steps:
+                raise SystemExit(f"{db}: {len(ssa)} k_ssa launches, need more than {steps}")
+            w0, w1 = ssa[-steps - 1][3], ssa[-1][3]
+            win = [r for r in rows if r[2] >= w0 and r[3] <= w1]
+        agg = collections.OrderedDict()
+        for r in sorted(win, key=lambda r: -r[4]):
+            a = agg.setdefault(_short_kernel(r[1]), {"n": 0, "ns": 0.0, "cnt": collections.defaultdict(float)})
+            a["n"] += 1
+            a["ns"] += r[4]
+            for k, v in counters.get(r[0], {}).items():
+                a["cnt"][k] += v
+        names = sorted({k for a in agg.values() for k in a["cnt"]})
+        print(f"# {db}: {len(rows)} dispatches; " +
+              ("copy calibration (whole run)" if calib else f"window = last {steps} steps, {len(win)} dispatches"))
+        hdr = f"{'kernel':60s} {'calls':>6s} {'avg us':>10s}" + "".join(f" {n + ' KiB':>18s}" for n in names)
+        print(hdr)
+        for k, a in sorted(agg.items(), key=lambda kv: -kv[1]["ns"]):
+            print(f"{k[:60]:60s} {a['n']:6d} {a['ns'] / a['n'] / 1e3:10.2f}" +
+                  "".join(f" {a['cnt'].get(n, 0.0) / a['n']:18.1f}" for n in names))
+        if calib:
+            cp = [a for k, a in agg.items() if "copyBuffer" in k or "copy" in k.lower()]
+            if cp:
+                a = max(cp, key=lambda a: a["ns"])
+                for n in names:
+                    per = a["cnt"].get(n, 0.0) / a["n"]
+                    if per > 0:
+                        scale[n] = (calibration_mib << 10) / per
+                        print(f"calibration {n}: {per:.0f} KiB per dispatch for {calibration_mib << 10} KiB copied "
+                              f"-> x{scale[n]:.3f}")
+        else:
+            tables.append((agg, names))
+        print()
+    if scale:
+        tot = collections.defaultdict(float)
+        for agg, names in tables:
+            for k, a in agg.items():
+                for n in names:
+                    if n in scale:
+                        tot[("ssa" if k.startswith("ssa::k_ssa") else "pp", n)] += a["cnt"].get(n, 0.0) * scale[n] * 1024
+        pp = sum(v for (g, _), v in tot.items() if g == "pp") / steps
+        ks = sum(v for (g, _), v in tot.items() if g == "ssa") / steps
+        print(f"scaled HBM bytes per step, particle path (every kernel but k_ssa): {pp:.4g}")
+        print(f"particle_path traffic_bytes_per_step={pp:.6g}")
+        print(f"k_ssa traffic_bytes_per_launch={ks:.6g}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -563,7 +650,16 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with torch (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--profile-summary", nargs="+", default=None, metavar="DB",
+                    help="only summarise rocprofv3 result databases of this script (per-kernel means over "
+                         "the last --steps steps; a copy-calibration database scales FETCH_SIZE/WRITE_SIZE "
+                         "into HBM bytes) and exit; needs no GPU")
+    ap.add_argument("--calibration-mib", type=int, default=1024,
+                    help="MiB copied by the --copy-calibration run summarised with --profile-summary")
     args = ap.parse_args()
+    if args.profile_summary:
+        return profile_summary(args.profile_summary, 20 if args.steps is None else args.steps,
+                               args.calibration_mib)
     if args.copy_calibration:
         import torch
         nb = args.copy_calibration << 20
