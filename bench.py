@@ -122,30 +122,83 @@ def cpu_baseline(sa, state, start_step, steps):
                       f"(the reference build was absent): {sec:.2f} s"}
 
 
-# matrix-core flops of one surrogate forward pass per node: encoder 32->64,
-# per layer A|B 64->128, W2 64->64, Wn1 128->64, Wn2 64->64 (x 3 layers)
-SURROGATE_FLOPS_PER_NODE = 2 * (32 * 64) + 3 * 2 * (64 * 128 + 64 * 64 + 128 * 64 + 64 * 64)
+# matrix-core flops of the surrogate's edge network per listed pair:
+# 16 features -> 64 -> 64 -> 4 (the products the fused kernel issues are
+# larger: activations enter as two bf16 halves, the first layer's K is
+# padded to 32 and the head's outputs to 16)
+SURROGATE_FLOPS_PER_EDGE = 2 * (16 * 64 + 64 * 64 + 64 * 4)
+SURROGATE_MFMA_FLOPS_PER_SLOT = 2 * 2 * (32 * 64 + 64 * 64 + 64 * 16)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_timing(eng, sa, plain_pair_ms):
-    """GPU time of one forward pass of the message-passing surrogate on the
-    same 1M-particle state (weights untrained: the timing does not depend on
-    them), next to the plain pair-force kernel it would replace."""
+def surrogate_leg(args, sa, E, plain_ms_per_step, plain_pair_ms):
+    """The surrogate on the same cylinder run: trained on the plain GPU run's
+    own steps (a sample of 32,768 moving particles every 25 steps of the
+    2,000-step run; feature scales from the state at the settle point),
+    then the run again with the network attached: a check against the plain
+    pair kernel every 50 steps with the 2% budget, re-armed (a failed check
+    pauses the network until the next check passes) and refreshed online
+    from each check's plain step (8 Adam steps).  Reports the training cost
+    and error, the checks' outcomes, the share of steps the network drove,
+    the timed window's ms/step with the network beside the plain run's, the
+    fused inference kernel against the plain pair kernel, and its
+    matrix-core rate."""
+    from spatialpy_amd.engine import Engine
     from spatialpy_amd.surrogate import Surrogate, fit_scales
+    K = 20 if args.steps is None else args.steps
+    W = 5 if args.warmup is None else args.warmup
+    first = args.settle + W
+    eng = Engine(sa)
+    eng.reset(seed=12345)
+    eng.advance(args.settle)
     s = Surrogate(seed=0)
     s.set_scales(*fit_scales(eng, sa))
-    idx = s.add_engine_sample(eng)
-    ms = s.time_forward(idx, reps=5)
-    tflops = SURROGATE_FLOPS_PER_NODE * sa.n / (ms * 1e-3) / 1e12
+    eng.reset(seed=12345)
+    s.record(eng, 25, max_particles=32768)
+    t = time.perf_counter()
+    eng.advance(CYL_RUN_STEPS)
+    eng.state()
+    rec_s = time.perf_counter() - t
+    s.stop_recording(eng)
+    n_samples = s.num_samples()
+    t = time.perf_counter()
+    loss = s.fit(args.surrogate_epochs, lr=3e-3, lr_end=3e-5)
+    train_s = time.perf_counter() - t
+    train_err = s.relative_error()
+    # the run with the network attached; the same window timed
+    eng.reset(seed=12345)
+    s.attach(eng, check_every=50, max_rel_err=0.02, refresh_iters=8, refresh_lr=1e-4, keep_samples=400,
+             max_particles=32768, rearm=True)
+    eng.advance(first)
+    eng.state()
+    t0 = time.perf_counter()
+    eng.advance(K)
+    eng.state()
+    win_s = time.perf_counter() - t0
+    st_win = Surrogate.stats(eng)
+    eng.advance(CYL_RUN_STEPS - first - K)
+    st = Surrogate.stats(eng)
+    infer_ms = s.time_engine(eng, reps=5)
     s.close()
-    return {"forward_ms": round(ms, 4), "plain_pair_kernel_ms": round(plain_pair_ms, 4),
-            "speed_vs_plain_pair_kernel": round(plain_pair_ms / ms, 4),
+    eng.close()
+    tflops = SURROGATE_FLOPS_PER_EDGE * E * sa.n / (infer_ms * 1e-3) / 1e12
+    return {"protocol": "trained on the plain 2,000-step run (a 32,768-particle sample every 25 steps), then the "
+                        "run again with a check against the plain pair kernel every 50 steps, 2% budget, re-armed, "
+                        "refreshed online from the checks",
+            "samples": n_samples, "record_run_s": round(rec_s, 2), "train_epochs": args.surrogate_epochs,
+            "train_s": round(train_s, 2), "train_loss": loss, "train_rel_err": round(train_err, 5),
+            "checks": st["checks"], "failed_checks": st["failed_checks"],
+            "max_passing_check_err": round(st["max_err"], 5), "last_check_err": round(st["last_err"], 5),
+            "nn_steps": st["nn_steps"], "nn_share": round(st["nn_steps"] / CYL_RUN_STEPS, 4),
+            "window_ms_per_step": round(1e3 * win_s / K, 4), "window_nn_steps": st_win["nn_steps"],
+            "plain_window_ms_per_step": round(plain_ms_per_step, 4),
+            "speed_vs_plain_step": round(plain_ms_per_step / (1e3 * win_s / K), 4),
+            "infer_ms": round(infer_ms, 4), "plain_pair_kernel_ms": round(plain_pair_ms, 4),
+            "speed_vs_plain_pair_kernel": round(plain_pair_ms / infer_ms, 4),
             "mfma": {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5),
-                     "flops_per_node": SURROGATE_FLOPS_PER_NODE},
-            "note": "bf16 MFMA GEMMs + FP32 edge sums; accuracy is measured by the GPU tests "
-                    "(tests/test_gpu_mpn.py), not here"}
+                     "flops_per_edge": SURROGATE_FLOPS_PER_EDGE,
+                     "issued_mfma_flops_per_list_slot": SURROGATE_MFMA_FLOPS_PER_SLOT}}
 
 
 def ssa_bytes_per_event(S, R, E):
@@ -524,9 +577,9 @@ def bench_cylinder(args):
     }
     if full is not None:
         out["full_run"] = full
-    if not args.no_surrogate:
-        out["surrogate"] = surrogate_timing(eng, sa, dph["forces"][0] / K)
     eng.close()
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(args, sa, E, 1e3 * sec / K, dph["forces"][0] / K)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sa, state, first, cpu_steps)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
@@ -646,7 +699,9 @@ def main():
                     help="cylinder: stop after the timed window instead of finishing the 2,000-step run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
-    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate forward-pass timing")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate leg (training on the plain run, the run with the network attached)")
+    ap.add_argument("--surrogate-epochs", type=int, default=150, help="Adam epochs over the recorded samples")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with torch (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")

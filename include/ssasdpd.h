@@ -292,44 +292,66 @@ int ssasdpd_ensemble_run(ssasdpd_ensemble *m, const uint64_t *seeds, uint32_t *t
 int ssasdpd_ensemble_stats(ssasdpd_ensemble *m, ssasdpd_stats *s);
 
 /* ---- Learned surrogate for the SDPD accelerations ------------------------
- * No counterpart in the reference.  A message-passing network over the
- * neighbour graph (3 layers; edge MLP 132->64->64 on [h_i, h_j, dx/h, r/h]
- * summed over neighbours, node MLP 128->64->64, hidden width 64, SiLU, bf16
- * weights, FP32 accumulation on the matrix cores) predicts each moving
- * particle's pair acceleration F - g.  Node features: v / v_scale,
- * (rho/rho0 - 1) / rho_scale, solid flag, nu / nu_scale, 1, (vt - v) / v_scale.  Training runs
- * on the GPU (forward, backward, Adam); the engine uses the network between
- * periodic checks against the plain SDPD and falls back to it for the rest
- * of the run when the relative error exceeds the budget.  Reactions and
- * diffusion stay exact SSA. */
+ * No counterpart in the reference.  An edge network over the neighbour
+ * lists predicts each moving particle's pair acceleration F - g (the F sum
+ * of the pair loop): per listed pair (i, j) a 3-layer MLP (16 rotation-
+ * invariant features -> 64 -> 64 -> 4, hard-SiLU, bf16 weights, FP32
+ * accumulation on the matrix cores) gives the coefficients of dx/h,
+ * dv/v_scale, v_i/v_scale and v_j/v_scale, times the envelope
+ * (1-R)^2 R/(R+0.001), R = r/h; the prediction is their sum over the
+ * neighbours, in units of acc_scale.  Training runs on the GPU (forward,
+ * backward, Adam); the engine uses the network between periodic checks
+ * against the plain SDPD and falls back to it for the rest of the run when
+ * the relative error exceeds the budget.  Reactions and diffusion stay
+ * exact SSA. */
 typedef struct ssasdpd_mpn ssasdpd_mpn;
+/* A particle state and its neighbour lists, host arrays: x, v, vt [n][3];
+ * rho, mass, nu [n]; solid [n] (nonzero: fixed); ELL lists deg [n],
+ * nidx [W][n] and nr [W][n] (pair distances), cut-off h. */
+typedef struct {
+    int32_t n, W, dim;
+    double h;
+    const double *x, *v, *vt, *rho, *mass, *nu;
+    const int32_t *solid, *deg, *nidx;
+    const double *nr;
+} ssasdpd_mpn_graph;
 int32_t ssasdpd_mpn_num_params(void);
 int ssasdpd_mpn_create(uint64_t seed, ssasdpd_mpn **out);
 void ssasdpd_mpn_destroy(ssasdpd_mpn *m);
 int ssasdpd_mpn_get_params(ssasdpd_mpn *m, float *params);
 int ssasdpd_mpn_set_params(ssasdpd_mpn *m, const float *params);
-/* scales[5] = {rho0, v_scale, rho_scale, nu_scale, acc_scale} */
+/* scales[6] = {rho0, v_scale, rho_scale, nu_scale, acc_scale, tv_scale}
+ * (tv_scale: |v - vt|); all positive.  Set before adding samples. */
 int ssasdpd_mpn_set_scales(ssasdpd_mpn *m, const float *scales);
-/* One training graph: x [n][3], node features X0 [n][32], ELL lists deg [n]
- * and nidx [W][n], targets y [n][3] (normalised F - g). */
-int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, int32_t n, int32_t W, float inv_h, const float *x, const float *X0,
-                           const int32_t *deg, const int32_t *nidx, const float *y, int32_t *index);
-/* A training graph from an engine's current state, with the plain pair
- * kernel's accelerations on that state as the target. */
-int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t *index);
+/* A training sample from a host graph: the n_sel particles sel[] (NULL: all
+ * moving particles) with targets acc [n][3] (pair accelerations F - g). */
+int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, const ssasdpd_mpn_graph *g, const double *acc, int32_t n_sel,
+                           const int32_t *sel, int32_t *index);
+/* A training sample from an engine's current state: up to max_particles
+ * moving particles (a seeded uniform subset), the plain pair kernel's
+ * accelerations on that state as the target. */
+int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t max_particles, uint64_t seed,
+                                  int32_t *index);
 int ssasdpd_mpn_clear_samples(ssasdpd_mpn *m);
-/* Record a training graph every `every` steps of the engine's plain SDPD
- * steps (m NULL: stop): the state the pair kernel sees inside the step
- * (after the predictor) and its accelerations.  Scales must be set first. */
-int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every);
+/* Record a sample (up to max_particles particles) every `every` plain SDPD
+ * steps of the engine (m NULL: stop): the state the pair kernel sees inside
+ * the step (after the predictor) and its accelerations.  Scales first. */
+int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every, int32_t max_particles);
 int32_t ssasdpd_mpn_num_samples(ssasdpd_mpn *m);
-/* Network output [n][3] (normalised) for a stored graph. */
-int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t sample, float *out);
-/* GPU time of one forward pass over a stored graph (HIP events, reps runs). */
-int ssasdpd_mpn_time_forward(ssasdpd_mpn *m, int32_t sample, int32_t reps, double *ms_per_forward);
-/* Mean squared error of a stored graph and its gradient [num_params]. */
+int ssasdpd_mpn_sample_size(ssasdpd_mpn *m, int32_t sample, int32_t *n_particles, int32_t *n_edges);
+/* Training-path predictions out [n_particles][3] and the sample's targets
+ * [n_particles][3] (either may be NULL), pair accelerations. */
+int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t sample, float *out, float *target);
+/* The fused inference kernel on a host graph: acc [n][3] pair accelerations
+ * (0 for solid particles and for components past dim). */
+int ssasdpd_mpn_predict(ssasdpd_mpn *m, const ssasdpd_mpn_graph *g, double *acc);
+/* GPU time of one fused inference pass over an engine's current state
+ * (HIP events over reps passes on the engine's stream). */
+int ssasdpd_mpn_time_engine(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t reps, double *ms_per_pass);
+/* Loss (1/n_particles) sum |pred - target|^2 in units of acc_scale^2 of a
+ * stored sample and its gradient [num_params]. */
 int ssasdpd_mpn_grad(ssasdpd_mpn *m, int32_t sample, float *grad, double *loss);
-/* Adam, one step per stored graph per epoch (shuffled); mean loss of the
+/* Adam, one step per stored sample per epoch (shuffled); mean loss of the
  * last epoch. */
 int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle_seed, double *mean_loss);
 /* Attach (m != NULL) or detach the surrogate: on moving domains every step
@@ -338,8 +360,25 @@ int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle
  * over moving particles), and an error above max_rel_err switches the
  * network off until the next reset or attach.  Counters restart at 0. */
 int ssasdpd_set_surrogate(ssasdpd_engine *e, ssasdpd_mpn *m, int32_t check_every, double max_rel_err);
+/* Online refresh while the surrogate is attached (iters 0: off, the
+ * default): at every check that passes, the plain step's state becomes a
+ * training sample of up to max_particles moving particles, samples beyond
+ * the newest keep_samples are dropped, and the network takes `iters` Adam
+ * steps (learning rate lr), alternately on the new sample and on one drawn
+ * from all stored samples, so it follows the run's own states without
+ * forgetting the recorded ones. */
+int ssasdpd_set_surrogate_refresh(ssasdpd_engine *e, int32_t iters, float lr, int32_t keep_samples,
+                                  int32_t max_particles);
+/* Re-arm (on != 0; off by default): a failed check switches the network
+ * off only until the next check, which still runs and switches it back on
+ * if it passes, so every interval the network drives starts from a passing
+ * check.  Off: the first failure ends its use for the run. */
+int ssasdpd_set_surrogate_rearm(ssasdpd_engine *e, int32_t on);
+/* Since the last reset or attach: steps that took F from the network,
+ * checks, the last check's error, the largest error of a passing check,
+ * online refreshes, failed checks, and whether the network is on. */
 int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *checks, double *last_err,
-                            int32_t *active);
+                            double *max_err, uint64_t *refreshes, uint64_t *failed_checks, int32_t *active);
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 

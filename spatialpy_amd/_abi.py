@@ -95,6 +95,12 @@ class State(C.Structure):
                 ("rrate", _dp), ("Ddiag", _dp), ("next_time", _dp)]
 
 
+class MpnGraph(C.Structure):
+    _fields_ = [("n", C.c_int32), ("W", C.c_int32), ("dim", C.c_int32), ("h", C.c_double),
+                ("x", _dp), ("v", _dp), ("vt", _dp), ("rho", _dp), ("mass", _dp), ("nu", _dp),
+                ("solid", _ip), ("deg", _ip), ("nidx", _ip), ("nr", _dp)]
+
+
 class Event(C.Structure):
     _fields_ = [("t", C.c_double), ("voxel", C.c_int32), ("kind", C.c_int32),
                 ("which", C.c_int32), ("dest", C.c_int32)]
@@ -160,18 +166,22 @@ def _declare(lib):
     lib.ssasdpd_mpn_get_params.argtypes = [P, _fp]
     lib.ssasdpd_mpn_set_params.argtypes = [P, _fp]
     lib.ssasdpd_mpn_set_scales.argtypes = [P, _fp]
-    lib.ssasdpd_mpn_add_sample.argtypes = [P, C.c_int32, C.c_int32, C.c_float, _fp, _fp, _ip, _ip, _fp, _ip]
-    lib.ssasdpd_mpn_add_engine_sample.argtypes = [P, P, _ip]
+    lib.ssasdpd_mpn_add_sample.argtypes = [P, C.POINTER(MpnGraph), _dp, C.c_int32, _ip, _ip]
+    lib.ssasdpd_mpn_add_engine_sample.argtypes = [P, P, C.c_int32, C.c_uint64, _ip]
     lib.ssasdpd_mpn_clear_samples.argtypes = [P]
-    lib.ssasdpd_mpn_forward.argtypes = [P, C.c_int32, _fp]
+    lib.ssasdpd_mpn_sample_size.argtypes = [P, C.c_int32, _ip, _ip]
+    lib.ssasdpd_mpn_forward.argtypes = [P, C.c_int32, _fp, _fp]
+    lib.ssasdpd_mpn_predict.argtypes = [P, C.POINTER(MpnGraph), _dp]
+    lib.ssasdpd_mpn_time_engine.argtypes = [P, P, C.c_int32, _dp]
     lib.ssasdpd_mpn_grad.argtypes = [P, C.c_int32, _fp, _dp]
     lib.ssasdpd_mpn_train.argtypes = [P, C.c_int32, C.c_float, C.c_uint64, _dp]
     lib.ssasdpd_set_surrogate.argtypes = [P, P, C.c_int32, C.c_double]
-    lib.ssasdpd_mpn_record.argtypes = [P, P, C.c_int32]
-    lib.ssasdpd_mpn_time_forward.argtypes = [P, C.c_int32, C.c_int32, _dp]
+    lib.ssasdpd_mpn_record.argtypes = [P, P, C.c_int32, C.c_int32]
     lib.ssasdpd_mpn_num_samples.argtypes = [P]
     lib.ssasdpd_mpn_num_samples.restype = C.c_int32
-    lib.ssasdpd_surrogate_stats.argtypes = [P, _u64p, _u64p, _dp, _ip]
+    lib.ssasdpd_surrogate_stats.argtypes = [P, _u64p, _u64p, _dp, _dp, _u64p, _u64p, _ip]
+    lib.ssasdpd_set_surrogate_rearm.argtypes = [P, C.c_int32]
+    lib.ssasdpd_set_surrogate_refresh.argtypes = [P, C.c_int32, C.c_float, C.c_int32, C.c_int32]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
                  "ssasdpd_run", "ssasdpd_run_batch", "ssasdpd_stats_get", "ssasdpd_get_state",
                  "ssasdpd_get_neighbors", "ssasdpd_fr_search", "ssasdpd_set_trace_capacity",
@@ -182,7 +192,8 @@ def _declare(lib):
                  "ssasdpd_mpn_set_params", "ssasdpd_mpn_set_scales", "ssasdpd_mpn_add_sample",
                  "ssasdpd_mpn_add_engine_sample", "ssasdpd_mpn_clear_samples", "ssasdpd_mpn_forward",
                  "ssasdpd_mpn_grad", "ssasdpd_mpn_train", "ssasdpd_set_surrogate", "ssasdpd_surrogate_stats",
-                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_forward", "ssasdpd_vm_eval", "ssasdpd_set_timeout", "ssasdpd_fr_search_noalloc"]:
+                 "ssasdpd_mpn_record", "ssasdpd_mpn_time_engine", "ssasdpd_mpn_sample_size",
+                 "ssasdpd_mpn_predict", "ssasdpd_set_surrogate_refresh", "ssasdpd_set_surrogate_rearm", "ssasdpd_vm_eval", "ssasdpd_set_timeout", "ssasdpd_fr_search_noalloc"]:
         getattr(lib, name).restype = C.c_int
     return lib
 

@@ -183,6 +183,7 @@ struct ssasdpd_engine {
     bool rdme_initialized = false;
     ssasdpd_stats stats{};
     int max_deg_seen = 0;
+    int last_max_deg = 0;       // longest list of the last search
     // GPU time per phase, HIP events on the engine stream: 0 binning + tree,
     // 1 search + D_ij, 2 SDPD substeps, 3 rates + queue build, 4 SSA loop,
     // 5 force kernel alone
@@ -221,7 +222,7 @@ struct ssasdpd_engine {
     ~ssasdpd_engine() { release(); }
 
     void release() {
-        dfree(nn_X0); dfree(nn_xf); dfree(nn_F); dfree(nn_acc);
+        dfree(nn_F); dfree(nn_acc);
         dfree(b_xx); dfree(b_rr); dfree(b_sr); dfree(b_sd); dfree(b_qt); dfree(b_tmin); dfree(b_targ);
         dfree(b_qs); dfree(b_seeds);
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
@@ -280,57 +281,64 @@ struct ssasdpd_engine {
     // surrogate for the pair accelerations (null: plain SDPD)
     Mpn *nn = nullptr;
     int nn_check_every = 50;
-    double nn_tol = 0.02, nn_last_err = -1.0;
+    double nn_tol = 0.02, nn_last_err = -1.0, nn_max_err = 0.0;
     bool nn_active = false;
     uint64_t nn_steps = 0, nn_checks = 0;
-    float *nn_X0 = nullptr, *nn_xf = nullptr;
-    Mpn *nn_rec = nullptr;         // record training graphs during plain steps
-    int nn_rec_every = 0;
+    int nn_refresh_iters = 0, nn_refresh_keep = 32;
+    float nn_refresh_lr = 1e-4f;
+    uint64_t nn_refreshes = 0, nn_fails = 0;
+    bool nn_rearm = false;
+    Mpn *nn_rec = nullptr;         // record training samples during plain steps
+    int nn_rec_every = 0, nn_rec_max = 32768;
+    uint64_t nn_rec_seed = 1;
     double *nn_F = nullptr, *nn_acc = nullptr;
     int nn_cap = 0;
+    std::vector<int> movers;       // particles that are not fixed
 
-    MpnGraph nn_graph() {
-        MpnGraph gr;
-        gr.n = n; gr.W = W; gr.inv_h = (float)(1.0 / h); gr.x = nn_xf; gr.deg = deg; gr.nidx = nidx;
+    // the network's view of the current state: the gather records of the
+    // predictor (or of eval_forces) and this step's lists
+    EcnGraph nn_graph() {
+        EcnGraph gr;
+        gr.n = n; gr.W = W; gr.dim = dim; gr.h = h; gr.rec = rec; gr.deg = deg; gr.nidx = nidx; gr.nr = nr;
         return gr;
     }
     void nn_ensure() {
         if (nn_cap >= n) return;
-        dfree(nn_X0); dfree(nn_xf); dfree(nn_F); dfree(nn_acc);
-        nn_X0 = dalloc<float>((size_t)n * 32);
-        nn_xf = dalloc<float>((size_t)n * 3);
+        dfree(nn_F); dfree(nn_acc);
         nn_F = dalloc<double>((size_t)n * 3);
         nn_acc = dalloc<double>(2);
         nn_cap = n;
     }
-    // A training graph from the state the force kernel just saw (after the
-    // predictor, lists of this step) with its accelerations as the target.
-    void record_sample() {
-        Mpn *M = nn_rec;
-        MpnSample smp;
-        smp.n = n; smp.W = W; smp.inv_h = (float)(1.0 / h);
-        smp.x = dalloc<float>((size_t)n * 3); smp.X0 = dalloc<float>((size_t)n * 32);
-        smp.deg = dalloc<int>(n); smp.nidx = dalloc<int>((size_t)n * W); smp.y = dalloc<float>((size_t)n * 3);
-        mpn_features(st, n, x, v, vt, rho, nu, solid, M->rho0, M->v_scale, M->rho_scale, M->nu_scale, smp.X0, smp.x);
-        mpn_target(st, n, F, solid, g, M->acc_scale, smp.y);
-        HIP_OK(hipMemcpyAsync(smp.deg, deg, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-        HIP_OK(hipMemcpyAsync(smp.nidx, nidx, sizeof(int) * (size_t)n * W, hipMemcpyDeviceToDevice, st));
+    // up to maxp moving particles, a uniform random subset in index order
+    std::vector<int> pick_particles(int maxp, uint64_t seed) const {
+        std::vector<int> sel = movers;
+        if ((int)sel.size() > maxp) {
+            std::mt19937_64 rng(seed);
+            for (int k = 0; k < maxp; k++) {
+                std::uniform_int_distribution<size_t> u(k, sel.size() - 1);
+                std::swap(sel[k], sel[u(rng)]);
+            }
+            sel.resize(maxp);
+            std::sort(sel.begin(), sel.end());
+        }
+        return sel;
+    }
+    // A training sample from the state the force kernel just saw (after the
+    // predictor, lists of this step) with its accelerations F as the target.
+    int record_sample(Mpn *M, int maxp, uint64_t seed) {
+        std::vector<int> dg(n);
+        d2h(st, dg.data(), deg, n);
         HIP_OK(hipStreamSynchronize(st));
-        M->samples.push_back(smp);
+        const std::vector<int> sel = pick_particles(maxp, seed);
+        if (sel.empty()) return -1;
+        return M->add_sample(st, nn_graph(), dg.data(), sel.data(), (int)sel.size(), nullptr, F, g);
     }
 
     // the network's accelerations for the current state into Fout (SoA
-    // [3][n]); the caller has run nn_ensure(), so nn_F is allocated before it
-    // is passed here
+    // [3][n])
     void nn_predict(double *Fout) {
-        if (!Fout || nn_cap < n)
-            throw HipError("surrogate buffers not allocated", hipErrorInvalidValue, __FILE__, __LINE__);
-        mpn_features(st, n, x, v, vt, rho, nu, solid, nn->rho0, nn->v_scale, nn->rho_scale, nn->nu_scale, nn_X0,
-                     nn_xf);
-        HIP_OK(hipStreamSynchronize(st));
-        nn->forward(nn_graph(), nn_X0, nullptr);
-        HIP_OK(hipStreamSynchronize(nn->st));
-        mpn_apply(st, n, dim, nn->out_buf(), solid, g, nn->acc_scale, Fout);
+        if (!Fout) throw HipError("surrogate output buffer missing", hipErrorInvalidValue, __FILE__, __LINE__);
+        nn->infer(st, nn_graph(), g, Fout);
     }
 
     // Level-1 nodes held in the event loop's LDS (with the levels above:
@@ -429,6 +437,9 @@ struct ssasdpd_engine {
         h2d(st, type, ty.data(), n);
         h2d(st, solid, so.data(), n);
         any_solid = std::any_of(so.begin(), so.end(), [](int v) { return v != 0; });
+        movers.clear();
+        for (int i = 0; i < n; i++)
+            if (!so[i]) movers.push_back(i);
         h2d(st, mass, d->mass, n);
         x0 = to_soa3(d->x, n);
         v0 = to_soa3(d->v, n);
@@ -567,6 +578,9 @@ struct ssasdpd_engine {
         nn_active = nn != nullptr;
         nn_steps = nn_checks = 0;
         nn_last_err = -1.0;
+        nn_max_err = 0.0;
+        nn_refreshes = 0;
+        nn_fails = 0;
         for (int p = 0; p < NPH; p++) { ph_ms[p] = 0; ph_launch[p] = 0; pending[p][0] = pending[p][1] = false; }
     }
 
@@ -618,9 +632,11 @@ struct ssasdpd_engine {
             // writes, and measured slower); each sorted list is finished (r,
             // D_ij with the current rho, rows) in the same kernel
             const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr};
+            // LDS list slots sized by the previous search's longest list
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
-                      deg, derr, nullptr, &fa);
+                      deg, derr, nullptr, &fa, last_max_deg);
             DevErr e = read_err();
+            last_max_deg = e.max_deg;
             max_deg_seen = std::max(max_deg_seen, e.max_deg);
             if (!e.overflow) break;
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
@@ -723,9 +739,11 @@ struct ssasdpd_engine {
             sa = sdpd_args(step);
         }
         ph_begin(5);
-        const bool nn_step = nn && nn_active && step > 0 && static_domain == 0;
-        if (nn_step) nn_ensure();
-        if (nn_step && step % nn_check_every != 0) {
+        // attached: checks run while the network is on, or always when a
+        // failed check only pauses it until the next passing one (re-arm)
+        const bool nn_on = nn && step > 0 && static_domain == 0 && (nn_active || nn_rearm);
+        if (nn_on) nn_ensure();
+        if (nn_on && nn_active && step % nn_check_every != 0) {
             // surrogate step: the pair kernel supplies Fbp, Frho and Q only,
             // the network the accelerations
             sa.skip_F = 1;
@@ -735,19 +753,34 @@ struct ssasdpd_engine {
             nn_steps++;
         } else {
             sdpd_forces(st, sa);                   // substep 1
-            if (nn_rec && step > 0 && static_domain == 0 && step % nn_rec_every == 0) record_sample();
-            if (nn_step) {
-                // periodic check against the plain SDPD; fall back for the
-                // rest of the run when the relative error exceeds the budget
+            if (nn_rec && step > 0 && static_domain == 0 && step % nn_rec_every == 0)
+                record_sample(nn_rec, nn_rec_max, nn_rec_seed++);
+            if (nn_on && step % nn_check_every == 0) {
+                // periodic check against the plain SDPD: above the budget the
+                // network is switched off (for the rest of the run, or until
+                // a check passes again with re-arm)
                 nn_predict(nn_F);
                 HIP_OK(hipMemsetAsync(nn_acc, 0, sizeof(double) * 2, st));
-                mpn_err(st, n, F, nn_F, solid, g, nn_acc);
+                ecn_err(st, n, F, nn_F, solid, g, nn_acc);
                 double acc[2];
                 d2h(st, acc, nn_acc, 2);
                 HIP_OK(hipStreamSynchronize(st));
                 nn_last_err = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
                 nn_checks++;
-                if (!(nn_last_err <= nn_tol)) nn_active = false;
+                nn_active = nn_last_err <= nn_tol;
+                if (nn_active)
+                    nn_max_err = std::max(nn_max_err, nn_last_err);
+                else
+                    nn_fails++;
+                if (nn_refresh_iters > 0) {
+                    // online refresh: this plain step becomes a sample (the
+                    // oldest beyond the window are dropped) and the network
+                    // takes a few Adam steps
+                    record_sample(nn, nn_rec_max, nn_rec_seed++);
+                    while ((int)nn->samples.size() > nn_refresh_keep) nn->drop_oldest();
+                    nn->train_steps(nn_refresh_iters, nn_refresh_lr, nn_rec_seed++);
+                    nn_refreshes++;
+                }
             }
         }
         ph_end(5);
@@ -1773,56 +1806,132 @@ int ssasdpd_mpn_set_params(ssasdpd_mpn *m, const float *w) {
 }
 
 int ssasdpd_mpn_set_scales(ssasdpd_mpn *m, const float *sc) {
-    m->m->rho0 = sc[0]; m->m->v_scale = sc[1]; m->m->rho_scale = sc[2]; m->m->nu_scale = sc[3];
-    m->m->acc_scale = sc[4];
+    for (int k = 0; k < 6; k++)
+        if (!(sc[k] > 0.f) || !std::isfinite(sc[k]))
+            return set_error(SSASDPD_ERR_BAD_DESC, "mpn scales must be positive and finite");
+    EcnScales &s = m->m->sc;
+    s.rho0 = sc[0]; s.vs = sc[1]; s.rs = sc[2]; s.ns = sc[3]; s.acc = sc[4]; s.ts = sc[5];
     return 0;
 }
 
-int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, int32_t n, int32_t W, float inv_h, const float *x, const float *X0,
-                           const int32_t *deg, const int32_t *nidx, const float *y, int32_t *index) {
-    GUARD_BEGIN
-    if (n <= 0 || W <= 0) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: empty graph");
+// A host-supplied particle graph on the device (records and lists).
+namespace {
+struct DevGraph {
+    PRec *rec = nullptr;
+    int *deg = nullptr, *nidx = nullptr;
+    double *nr = nullptr;
+    EcnGraph view;
+    std::vector<int> deg_h, movers;
+    ~DevGraph() { dfree(rec); dfree(deg); dfree(nidx); dfree(nr); }
+};
+int upload_graph(const ssasdpd_mpn_graph *g, hipStream_t st, DevGraph &d) {
+    if (!g || g->n <= 0 || g->W <= 0 || g->dim < 1 || g->dim > 3 || !(g->h > 0.0) || !g->x || !g->v || !g->vt ||
+        !g->rho || !g->mass || !g->nu || !g->solid || !g->deg || !g->nidx || !g->nr)
+        return set_error(SSASDPD_ERR_BAD_DESC, "mpn graph: bad arguments");
+    const int n = g->n, W = g->W;
     for (int i = 0; i < n; i++) {
-        if (deg[i] < 0 || deg[i] > W) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: degree out of range");
-        for (int k = 0; k < deg[i]; k++) {
-            const int j = nidx[(size_t)k * n + i];
-            if (j < 0 || j >= n) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: neighbour index out of range");
+        if (g->deg[i] < 0 || g->deg[i] > W) return set_error(SSASDPD_ERR_BAD_DESC, "mpn graph: degree out of range");
+        for (int k = 0; k < g->deg[i]; k++) {
+            const int j = g->nidx[(size_t)k * n + i];
+            if (j < 0 || j >= n) return set_error(SSASDPD_ERR_BAD_DESC, "mpn graph: neighbour index out of range");
         }
+        if (!(g->mass[i] > 0.0) || !(g->rho[i] > 0.0))
+            return set_error(SSASDPD_ERR_BAD_DESC, "mpn graph: mass and density must be positive");
     }
+    std::vector<PRec> r(n);
+    for (int i = 0; i < n; i++) {
+        PRec &q = r[i];
+        std::memset(&q, 0, sizeof(PRec));
+        for (int c = 0; c < 3; c++) {
+            q.x[c] = g->x[3 * (size_t)i + c];
+            q.v[c] = g->v[3 * (size_t)i + c];
+            q.vt[c] = g->vt[3 * (size_t)i + c];
+        }
+        q.nu = g->nu[i]; q.rho = g->rho[i]; q.mass = g->mass[i]; q.type = 1; q.solid = g->solid[i] ? 1 : 0;
+        if (!q.solid) d.movers.push_back(i);
+    }
+    d.deg_h.assign(g->deg, g->deg + n);
+    d.rec = dalloc<PRec>(n);
+    d.deg = dalloc<int>(n);
+    d.nidx = dalloc<int>((size_t)W * n);
+    d.nr = dalloc<double>((size_t)W * n);
+    h2d(st, d.rec, r.data(), n);
+    h2d(st, d.deg, g->deg, n);
+    h2d(st, d.nidx, g->nidx, (size_t)W * n);
+    h2d(st, d.nr, g->nr, (size_t)W * n);
+    HIP_OK(hipStreamSynchronize(st));
+    d.view.n = n; d.view.W = W; d.view.dim = g->dim; d.view.h = g->h;
+    d.view.rec = d.rec; d.view.deg = d.deg; d.view.nidx = d.nidx; d.view.nr = d.nr;
+    return 0;
+}
+}  // namespace
+
+int ssasdpd_mpn_add_sample(ssasdpd_mpn *m, const ssasdpd_mpn_graph *g, const double *acc, int32_t n_sel,
+                           const int32_t *sel, int32_t *index) {
+    GUARD_BEGIN
     ssa::Mpn *M = m->m;
-    MpnSample s;
-    s.n = n; s.W = W; s.inv_h = inv_h;
-    s.x = dalloc<float>((size_t)n * 3); s.X0 = dalloc<float>((size_t)n * 32);
-    s.deg = dalloc<int>(n); s.nidx = dalloc<int>((size_t)n * W); s.y = dalloc<float>((size_t)n * 3);
-    h2d(M->st, s.x, x, (size_t)n * 3); h2d(M->st, s.X0, X0, (size_t)n * 32); h2d(M->st, s.deg, deg, n);
-    h2d(M->st, s.nidx, nidx, (size_t)n * W); h2d(M->st, s.y, y, (size_t)n * 3);
+    DevGraph d;
+    if (int rc = upload_graph(g, M->st, d)) return rc;
+    if (!acc) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: no target accelerations");
+    std::vector<int> s = sel ? std::vector<int>(sel, sel + std::max(n_sel, 0)) : d.movers;
+    if (s.empty()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: no particles selected");
+    for (int i : s)
+        if (i < 0 || i >= g->n) return set_error(SSASDPD_ERR_BAD_DESC, "mpn sample: selected index out of range");
+    std::vector<float> y(3 * s.size());
+    for (size_t p = 0; p < s.size(); p++)
+        for (int c = 0; c < 3; c++) y[3 * p + c] = (float)(acc[3 * (size_t)s[p] + c] / (double)M->sc.acc);
+    const int k = M->add_sample(M->st, d.view, d.deg_h.data(), s.data(), (int)s.size(), y.data(), nullptr, nullptr);
+    if (index) *index = k;
+    return 0;
+    GUARD_END
+}
+
+int ssasdpd_mpn_predict(ssasdpd_mpn *m, const ssasdpd_mpn_graph *g, double *acc) {
+    GUARD_BEGIN
+    ssa::Mpn *M = m->m;
+    DevGraph d;
+    if (int rc = upload_graph(g, M->st, d)) return rc;
+    if (!acc) return set_error(SSASDPD_ERR_BAD_DESC, "mpn predict: no output");
+    const size_t n = g->n;
+    double *F = dalloc<double>(3 * n);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    M->infer(M->st, d.view, zero, F);
+    std::vector<double> tmp(3 * n);
+    d2h(M->st, tmp.data(), F, 3 * n);
     HIP_OK(hipStreamSynchronize(M->st));
-    M->samples.push_back(s);
-    if (index) *index = (int32_t)M->samples.size() - 1;
+    dfree(F);
+    from_soa3(tmp, acc, (int)n);
     return 0;
     GUARD_END
 }
 
 int ssasdpd_mpn_clear_samples(ssasdpd_mpn *m) {
-    for (auto &s : m->m->samples) { dfree(s.x); dfree(s.X0); dfree(s.deg); dfree(s.nidx); dfree(s.y); }
-    m->m->samples.clear();
+    m->m->clear_samples();
     return 0;
 }
 
-static MpnGraph sample_graph(const MpnSample &s) {
-    MpnGraph gr;
-    gr.n = s.n; gr.W = s.W; gr.inv_h = s.inv_h; gr.x = s.x; gr.deg = s.deg; gr.nidx = s.nidx;
-    return gr;
+int ssasdpd_mpn_sample_size(ssasdpd_mpn *m, int32_t idx, int32_t *n_particles, int32_t *n_edges) {
+    if (idx < 0 || idx >= (int)m->m->samples.size()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no such sample");
+    if (n_particles) *n_particles = m->m->samples[idx].np;
+    if (n_edges) *n_edges = m->m->samples[idx].ne;
+    return 0;
 }
 
-int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t idx, float *out) {
+int ssasdpd_mpn_forward(ssasdpd_mpn *m, int32_t idx, float *out, float *target) {
     GUARD_BEGIN
     ssa::Mpn *M = m->m;
     if (idx < 0 || idx >= (int)M->samples.size()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no such sample");
-    const MpnSample &s = M->samples[idx];
-    M->forward(sample_graph(s), s.X0, nullptr);
-    d2h(M->st, out, M->out_buf(), (size_t)s.n * 3);
+    const EcnSample &s = M->samples[idx];
+    M->forward(s, nullptr);
+    std::vector<float> p(3 * (size_t)s.np);
+    if (out) d2h(M->st, p.data(), M->pred(), p.size());
+    std::vector<float> y(3 * (size_t)s.np);
+    if (target) d2h(M->st, y.data(), s.y, y.size());
     HIP_OK(hipStreamSynchronize(M->st));
+    for (size_t q = 0; q < p.size(); q++) {
+        if (out) out[q] = p[q] * M->sc.acc;
+        if (target) target[q] = y[q] * M->sc.acc;
+    }
     return 0;
     GUARD_END
 }
@@ -1831,11 +1940,11 @@ int ssasdpd_mpn_grad(ssasdpd_mpn *m, int32_t idx, float *grad, double *loss) {
     GUARD_BEGIN
     ssa::Mpn *M = m->m;
     if (idx < 0 || idx >= (int)M->samples.size()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no such sample");
-    const MpnSample &s = M->samples[idx];
+    const EcnSample &s = M->samples[idx];
     float *dl = dalloc<float>(1);
     HIP_OK(hipMemsetAsync(dl, 0, sizeof(float), M->st));
-    M->forward(sample_graph(s), s.X0, nullptr);
-    M->backward(sample_graph(s), s.X0, s.y, dl);
+    M->forward(s, dl);
+    M->backward(s);
     float l = 0.f;
     d2h(M->st, &l, dl, 1);
     if (grad) d2h(M->st, grad, M->g, Mpn::P);
@@ -1851,6 +1960,7 @@ int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle
     ssa::Mpn *M = m->m;
     const int ns = (int)M->samples.size();
     if (ns == 0) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: no training samples");
+    if (epochs < 1 || !(lr > 0.f)) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: epochs >= 1 and lr > 0");
     float *dl = dalloc<float>(1);
     std::vector<int> order(ns);
     for (int i = 0; i < ns; i++) order[i] = i;
@@ -1860,76 +1970,86 @@ int ssasdpd_mpn_train(ssasdpd_mpn *m, int32_t epochs, float lr, uint64_t shuffle
         std::shuffle(order.begin(), order.end(), rng);
         HIP_OK(hipMemsetAsync(dl, 0, sizeof(float), M->st));
         for (int k = 0; k < ns; k++) {
-            const MpnSample &s = M->samples[order[k]];
-            M->forward(sample_graph(s), s.X0, nullptr);
-            M->backward(sample_graph(s), s.X0, s.y, dl);
+            const EcnSample &s = M->samples[order[k]];
+            M->forward(s, dl);
+            M->backward(s);
             M->adam(lr);
         }
-        float l = 0.f;
-        d2h(M->st, &l, dl, 1);
-        HIP_OK(hipStreamSynchronize(M->st));
-        last = l / ns;
+        if (ep == epochs - 1) {
+            float l = 0.f;
+            d2h(M->st, &l, dl, 1);
+            HIP_OK(hipStreamSynchronize(M->st));
+            last = l / ns;
+        }
     }
+    HIP_OK(hipStreamSynchronize(M->st));
     dfree(dl);
     if (mean_loss) *mean_loss = last;
     return 0;
     GUARD_END
 }
 
-int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t *index) {
+int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t max_particles, uint64_t seed,
+                                  int32_t *index) {
     GUARD_BEGIN
     if (!e->lists_built) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: engine has no neighbour lists yet");
-    ssa::Mpn *M = m->m;
-    const int n = e->n;
+    if (max_particles < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: max_particles must be >= 1");
+    if (e->movers.empty()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: the engine has no moving particles");
+    const size_t n = e->n, n3 = 3 * n;
     // the plain pair-force kernel on the current state is the target
-    std::vector<double> F(3 * (size_t)n);
-    int rc = ssasdpd_eval_forces(e, 0, F.data(), nullptr, nullptr);
-    if (rc) return rc;
-    e->nn_ensure();
-    mpn_features(e->st, n, e->x, e->v, e->vt, e->rho, e->nu, e->solid, M->rho0, M->v_scale, M->rho_scale,
-                 M->nu_scale, e->nn_X0, e->nn_xf);
-    std::vector<float> X0((size_t)n * 32), xf((size_t)n * 3), y((size_t)n * 3);
-    std::vector<int> dg(n), nx((size_t)n * e->W), sol(n);
-    d2h(e->st, X0.data(), e->nn_X0, X0.size());
-    d2h(e->st, xf.data(), e->nn_xf, xf.size());
+    double *dF = dalloc<double>(n3), *dB = dalloc<double>(n3), *dR = dalloc<double>(n);
+    std::vector<double> g0(n3);
+    for (int k = 0; k < 3; k++) std::fill(g0.begin() + k * n, g0.begin() + (k + 1) * n, e->g[k]);
+    h2d(e->st, dF, g0.data(), n3);
+    HIP_OK(hipMemsetAsync(dB, 0, sizeof(double) * n3, e->st));
+    HIP_OK(hipMemsetAsync(dR, 0, sizeof(double) * n, e->st));
+    SdpdArgs a = e->sdpd_args(e->cur);
+    a.F = dF; a.Fbp = dB; a.Frho = dR;
+    a.Sc = 0;
+    a.Rc = 0;
+    a.fp32 = 0;
+    sdpd_pack(e->st, a);
+    sdpd_forces(e->st, a);
+    std::vector<int> dg(n);
     d2h(e->st, dg.data(), e->deg, n);
-    d2h(e->st, nx.data(), e->nidx, nx.size());
-    d2h(e->st, sol.data(), e->solid, n);
     HIP_OK(hipStreamSynchronize(e->st));
-    for (int i = 0; i < n; i++)
-        for (int c = 0; c < 3; c++)
-            y[3 * (size_t)i + c] = sol[i] ? 0.f : (float)((F[3 * (size_t)i + c] - e->g[c]) / M->acc_scale);
-    return ssasdpd_mpn_add_sample(m, n, e->W, (float)(1.0 / e->h), xf.data(), X0.data(), dg.data(), nx.data(), y.data(),
-                                  index);
+    const std::vector<int> sel = e->pick_particles(max_particles, seed);
+    const int k = m->m->add_sample(e->st, e->nn_graph(), dg.data(), sel.data(), (int)sel.size(), nullptr, dF, e->g);
+    dfree(dF); dfree(dB); dfree(dR);
+    if (index) *index = k;
+    return 0;
     GUARD_END
 }
 
-int ssasdpd_mpn_time_forward(ssasdpd_mpn *m, int32_t idx, int32_t reps, double *ms_per_forward) {
+int ssasdpd_mpn_time_engine(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t reps, double *ms_per_pass) {
     GUARD_BEGIN
-    ssa::Mpn *M = m->m;
-    if (idx < 0 || idx >= (int)M->samples.size() || reps < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: bad sample/reps");
-    const MpnSample &s = M->samples[idx];
+    if (!e->lists_built || !e->rec_ready || reps < 1)
+        return set_error(SSASDPD_ERR_BAD_DESC, "mpn: engine has not stepped yet, or reps < 1");
+    e->nn_ensure();
     hipEvent_t a, b;
     HIP_OK(hipEventCreate(&a));
     HIP_OK(hipEventCreate(&b));
-    M->forward(sample_graph(s), s.X0, nullptr);
-    HIP_OK(hipEventRecord(a, M->st));
-    for (int r = 0; r < reps; r++) M->forward(sample_graph(s), s.X0, nullptr);
-    HIP_OK(hipEventRecord(b, M->st));
+    ssa::Mpn *M = m->m;
+    M->infer(e->st, e->nn_graph(), e->g, e->nn_F);
+    HIP_OK(hipEventRecord(a, e->st));
+    for (int r = 0; r < reps; r++) M->infer(e->st, e->nn_graph(), e->g, e->nn_F);
+    HIP_OK(hipEventRecord(b, e->st));
     HIP_OK(hipEventSynchronize(b));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    *ms_per_forward = ms / reps;
+    *ms_per_pass = ms / reps;
     return 0;
     GUARD_END
 }
 
-int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every) {
-    if (m && every < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn_record: every must be >= 1");
+int ssasdpd_mpn_record(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t every, int32_t max_particles) {
+    if (m && (every < 1 || max_particles < 1))
+        return set_error(SSASDPD_ERR_BAD_DESC, "mpn_record: every and max_particles must be >= 1");
     e->nn_rec = m ? m->m : nullptr;
     e->nn_rec_every = every;
+    if (m) e->nn_rec_max = max_particles;
     return 0;
 }
 
@@ -1943,14 +2063,36 @@ int ssasdpd_set_surrogate(ssasdpd_engine *e, ssasdpd_mpn *m, int32_t check_every
     e->nn_active = e->nn != nullptr;
     e->nn_steps = e->nn_checks = 0;
     e->nn_last_err = -1.0;
+    e->nn_max_err = 0.0;
+    e->nn_refreshes = 0;
+    e->nn_fails = 0;
+    return 0;
+}
+
+int ssasdpd_set_surrogate_refresh(ssasdpd_engine *e, int32_t iters, float lr, int32_t keep_samples,
+                                  int32_t max_particles) {
+    if (iters < 0 || (iters > 0 && (!(lr > 0.f) || keep_samples < 1 || max_particles < 1)))
+        return set_error(SSASDPD_ERR_BAD_DESC, "surrogate refresh: iters >= 0, lr > 0, keep and max_particles >= 1");
+    e->nn_refresh_iters = iters;
+    e->nn_refresh_lr = lr;
+    e->nn_refresh_keep = keep_samples;
+    if (iters > 0) e->nn_rec_max = max_particles;
+    return 0;
+}
+
+int ssasdpd_set_surrogate_rearm(ssasdpd_engine *e, int32_t on) {
+    e->nn_rearm = on != 0;
     return 0;
 }
 
 int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *checks, double *last_err,
-                            int32_t *active) {
+                            double *max_err, uint64_t *refreshes, uint64_t *failed_checks, int32_t *active) {
+    if (failed_checks) *failed_checks = e->nn_fails;
     if (nn_steps) *nn_steps = e->nn_steps;
     if (checks) *checks = e->nn_checks;
     if (last_err) *last_err = e->nn_last_err;
+    if (max_err) *max_err = e->nn_max_err;
+    if (refreshes) *refreshes = e->nn_refreshes;
     if (active) *active = e->nn_active ? 1 : 0;
     return 0;
 }

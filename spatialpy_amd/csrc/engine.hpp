@@ -155,7 +155,7 @@ struct FinishArgs {
 void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted,
                const int *cell_start, int n, int dim, double h, const Grid &g, const KdDev *kd,
                int W, int *out_idx, double *out_r2, int *deg, DevErr *err, const int *query_order,
-               const FinishArgs *fin = nullptr);
+               const FinishArgs *fin = nullptr, int slots_hint = 0);
 void kd_build(hipStream_t s, const double *pts, int n, int dim, KdDev &kd, void *work, size_t work_bytes);
 size_t kd_work_bytes(int n);
 size_t scan_tmp_bytes(long long n);
@@ -278,38 +278,5 @@ void vm_eval_batch(hipStream_t s, const uint32_t *code, int start, const double 
                    int n_df, int m, const double *xd, const uint32_t *xu, const double *vol, const double *t,
                    const double *df, double *out);
 size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG, int n_code = 0, int n_consts = 0);
-
-// ---- message-passing surrogate (mpn.hip) --------------------------------
-struct MpnGraph {
-    int n, W;
-    float inv_h;
-    const float *x;          // [n][3] positions, FP32
-    const int *deg, *nidx;   // ELL [W][n]
-};
-void mpn_gemm(hipStream_t s, int M, int K, int K1, const float *A0, int lda0, const float *A1, int lda1,
-              const __bf16 *Wt, int N, const float *bias, const int *rowscale, int act, const float *R, float *Z,
-              float *Y);
-void mpn_wgrad(hipStream_t s, int M, int K, int K1, const float *A0, int lda0, const float *A1, int lda1,
-               const float *dY, int ldy, int N, float *dW);
-void mpn_colsum(hipStream_t s, int M, int N, const float *dY, int ldy, const int *rowscale, float *db);
-void mpn_edge_fwd(hipStream_t s, const MpnGraph &g, const float *AB, const float *W1c, const float *b1, float *S);
-void mpn_edge_bwd(hipStream_t s, const MpnGraph &g, const float *AB, const float *W1c, const float *b1, const float *dS,
-                  float *dAB, float *dW1c, float *db1);
-void mpn_decode(hipStream_t s, int n, const float *H, const float *Wd, const float *bd, float *out);
-void mpn_decode_bwd(hipStream_t s, int n, const float *H, const float *Wd, const float *out, const float *y, float *dH,
-                    float *dWd, float *dbd, float *loss);
-void mpn_silu_bwd(hipStream_t s, size_t m, const float *dY, const float *Z, float *dZ);
-void mpn_add(hipStream_t s, int rows, int cols, float *y, const float *x, int ldx);
-void mpn_adam(hipStream_t s, int m, float *w, const float *g, float *mo, float *ve, float lr, int t);
-void mpn_to_bf16_t(hipStream_t s, int K, int Kp, int N, const float *W, __bf16 *Wt);
-void mpn_to_bf16(hipStream_t s, int K, int N, const float *W, __bf16 *Wb);
-void mpn_features(hipStream_t s, int n, const double *x, const double *v, const double *vt, const double *rho,
-                  const double *nu, const int *solid, float rho0, float v_scale, float rho_scale, float nu_scale,
-                  float *X0, float *xf);
-void mpn_target(hipStream_t s, int n, const double *F, const int *solid, const double *g, float acc_scale, float *y);
-void mpn_apply(hipStream_t s, int n, int dim, const float *out, const int *solid, const double *g, float acc_scale,
-               double *F);
-void mpn_err(hipStream_t s, int n, const double *Fref, const double *Fnn, const int *solid, const double *g,
-             double *acc);
 
 }  // namespace ssa

@@ -1,6 +1,7 @@
-// Host side of the message-passing surrogate (see mpn.hip for the network).
+// Host side of the learned surrogate (see mpn.hip for the network).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <random>
 #include <vector>
@@ -25,36 +26,27 @@ Mpn::Mpn(uint64_t seed) {
     wb = alloc<__bf16>(B_TOTAL);
     HIP_OK(hipMemsetAsync(m1, 0, sizeof(float) * P, st));
     HIP_OK(hipMemsetAsync(m2, 0, sizeof(float) * P, st));
-    // Glorot-uniform weights, zero biases, a small decoder
+    // He-normal hidden weights (W1 rows past the 16 features stay zero),
+    // small biases, a small head
     std::vector<float> h(P, 0.f);
     std::mt19937_64 rng(seed);
-    auto fill = [&](int off, int fan_in, int fan_out, int rows, float gain) {
-        const float a = gain * std::sqrt(6.0f / (float)(fan_in + fan_out));
-        std::uniform_real_distribution<float> u(-a, a);
-        for (int i = 0; i < rows * fan_out; i++) h[off + i] = u(rng);
+    auto normal = [&](int off, int count, float sd) {
+        std::normal_distribution<float> d(0.f, sd);
+        for (int i = 0; i < count; i++) h[off + i] = d(rng);
     };
-    fill(OFF_WE, 10, HID, 10, 1.f);  // rows past the 10 features stay 0
-    for (int l = 0; l < LAYERS; l++) {
-        const int b = L0 + l * LSIZE;
-        {   // W1ab [64][128] (A block | B block) and W1c [4][64]: the edge MLP's first layer, fan-in 132
-            const float a = std::sqrt(6.0f / (float)(2 * HID + 4 + HID));
-            std::uniform_real_distribution<float> u(-a, a);
-            for (int i = 0; i < HID * 128; i++) h[b + W1AB + i] = u(rng);
-            for (int i = 0; i < 4 * HID; i++) h[b + W1C + i] = u(rng);
-        }
-        fill(b + W2, HID, HID, HID, 1.f);
-        fill(b + WN1, 2 * HID, HID, 2 * HID, 1.f);
-        fill(b + WN2, HID, HID, HID, 0.5f);
-    }
-    fill(OFF_WD, HID, NOUT, HID, 0.1f);
+    normal(OFF_W1, NF * HID, std::sqrt(2.f / NF));
+    normal(OFF_B1, HID, 0.1f);
+    normal(OFF_W2, HID * HID, std::sqrt(2.f / HID));
+    normal(OFF_B2, HID, 0.1f);
+    normal(OFF_W3, HID * NOUT, 0.1f / std::sqrt((float)HID));
+    normal(OFF_L, NF * NOUT, 0.1f / std::sqrt((float)NF));
     set_params(h.data());
 }
 
 Mpn::~Mpn() {
     free_acts();
-    for (auto &s : samples) {
-        release(s.x); release(s.X0); release(s.deg); release(s.nidx); release(s.y);
-    }
+    clear_samples();
+    release(dloss_);
     release(w); release(g); release(m1); release(m2); release(wb);
     if (st) (void)hipStreamDestroy(st);
 }
@@ -69,106 +61,115 @@ void Mpn::get_params(float *host) {
     HIP_OK(hipStreamSynchronize(st));
 }
 
-void Mpn::refresh_bf16() {
-    mpn_to_bf16_t(st, NIN, NIN, HID, w + OFF_WE, wb + B_WE);
-    for (int l = 0; l < LAYERS; l++) {
-        const float *p = w + L0 + l * LSIZE;
-        __bf16 *q = wb + B_L0 + l * B_LSIZE;
-        mpn_to_bf16_t(st, HID, HID, 128, p + W1AB, q + B_W1T);
-        mpn_to_bf16(st, HID, 128, p + W1AB, q + B_W1);
-        mpn_to_bf16_t(st, HID, HID, HID, p + W2, q + B_W2T);
-        mpn_to_bf16(st, HID, HID, p + W2, q + B_W2);
-        mpn_to_bf16_t(st, 128, 128, HID, p + WN1, q + B_WN1T);
-        mpn_to_bf16(st, 128, HID, p + WN1, q + B_WN1);
-        mpn_to_bf16_t(st, HID, HID, HID, p + WN2, q + B_WN2T);
-        mpn_to_bf16(st, HID, HID, p + WN2, q + B_WN2);
+void Mpn::refresh_bf16() { ecn_bf16(st, w, wb); }
+
+void Mpn::infer(hipStream_t s, const EcnGraph &gr, const double *gvec, double *F) {
+    // the weights were last written on the network's own stream
+    HIP_OK(hipStreamSynchronize(st));
+    ecn_infer(s, gr, sc, wb, w, gvec, F);
+}
+
+void Mpn::clear_samples() {
+    for (auto &s : samples) {
+        release(s.X); release(s.Bv); release(s.pe); release(s.rowp); release(s.y);
     }
+    samples.clear();
+}
+
+int Mpn::add_sample(hipStream_t s, const EcnGraph &gr, const int *deg_host, const int *sel_host, int n_sel,
+                    const float *y_host, const double *F_dev, const double *gvec) {
+    std::vector<int> rowp(n_sel + 1, 0);
+    for (int p = 0; p < n_sel; p++) rowp[p + 1] = rowp[p] + deg_host[sel_host[p]];
+    EcnSample smp;
+    smp.np = n_sel;
+    smp.ne = rowp[n_sel];
+    smp.X = alloc<float>((size_t)smp.ne * KIN);
+    smp.Bv = alloc<float>((size_t)smp.ne * 16);
+    smp.pe = alloc<int>(smp.ne);
+    smp.rowp = alloc<int>(n_sel + 1);
+    smp.y = alloc<float>((size_t)n_sel * 3);
+    int *sel = alloc<int>(n_sel);
+    HIP_OK(hipMemcpyAsync(sel, sel_host, sizeof(int) * n_sel, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(smp.rowp, rowp.data(), sizeof(int) * (n_sel + 1), hipMemcpyHostToDevice, s));
+    if (y_host)
+        HIP_OK(hipMemcpyAsync(smp.y, y_host, sizeof(float) * 3 * n_sel, hipMemcpyHostToDevice, s));
+    else
+        ecn_target(s, gr.n, n_sel, sel, F_dev, gvec, sc.acc, smp.y);
+    ecn_edges(s, gr, sc, n_sel, sel, smp.rowp, smp.X, smp.Bv, smp.pe);
+    HIP_OK(hipStreamSynchronize(s));
+    release(sel);
+    samples.push_back(smp);
+    return (int)samples.size() - 1;
 }
 
 void Mpn::free_acts() {
-    release(Zenc_);
-    for (int l = 0; l <= LAYERS; l++) release(H_[l]);
-    for (int l = 0; l < LAYERS; l++) { release(AB_[l]); release(S_[l]); release(M_[l]); release(Z1_[l]); release(U_[l]); }
-    release(out_); release(dH_); release(dT_); release(dU_); release(dZ1_); release(dHM_); release(dS_); release(dAB_);
-    release(dZe_);
-    cap_ = 0;
+    release(Z1_); release(H1_); release(Z2_); release(H2_); release(O_); release(pred_); release(D_);
+    release(dH_); release(dZ_); release(part_);
+    cap_e_ = cap_p_ = 0;
 }
 
-void Mpn::ensure(int n) {
-    if (n <= cap_) return;
+void Mpn::ensure(size_t ne, size_t np) {
+    if (ne <= cap_e_ && np <= cap_p_) return;
     free_acts();
-    const size_t m = (size_t)n;
-    Zenc_ = alloc<float>(m * HID);
-    for (int l = 0; l <= LAYERS; l++) H_[l] = alloc<float>(m * HID);
-    for (int l = 0; l < LAYERS; l++) {
-        AB_[l] = alloc<float>(m * 128); S_[l] = alloc<float>(m * HID); M_[l] = alloc<float>(m * HID);
-        Z1_[l] = alloc<float>(m * HID); U_[l] = alloc<float>(m * HID);
-    }
-    out_ = alloc<float>(m * NOUT);
-    dH_ = alloc<float>(m * HID); dT_ = alloc<float>(m * HID); dU_ = alloc<float>(m * HID);
-    dZ1_ = alloc<float>(m * HID); dHM_ = alloc<float>(m * 128); dS_ = alloc<float>(m * HID);
-    dAB_ = alloc<float>(m * 128); dZe_ = alloc<float>(m * HID);
-    cap_ = n;
+    cap_e_ = ne; cap_p_ = np;
+    Z1_ = alloc<float>(ne * HID); H1_ = alloc<float>(ne * HID); Z2_ = alloc<float>(ne * HID);
+    H2_ = alloc<float>(ne * HID); O_ = alloc<float>(ne * NOUT);
+    dH_ = alloc<float>(ne * HID); dZ_ = alloc<float>(ne * HID);
+    pred_ = alloc<float>(np * 3); D_ = alloc<float>(np * 3);
+    const size_t parts = std::max({ecn_wgrad_parts((int)ne, HID, HID), (size_t)256 * HID, ecn_head_parts((int)ne)});
+    part_ = alloc<float>(parts);
 }
 
-void Mpn::forward(const MpnGraph &gr, const float *X0, float *out) {
-    const int n = gr.n;
-    ensure(n);
-    mpn_gemm(st, n, NIN, NIN, X0, NIN, nullptr, 0, wb + B_WE, HID, w + OFF_BE, nullptr, 1, nullptr, Zenc_, H_[0]);
-    for (int l = 0; l < LAYERS; l++) {
-        const float *p = w + L0 + l * LSIZE;
-        const __bf16 *q = wb + B_L0 + l * B_LSIZE;
-        mpn_gemm(st, n, HID, HID, H_[l], HID, nullptr, 0, q + B_W1T, 128, nullptr, nullptr, 0, nullptr, nullptr, AB_[l]);
-        mpn_edge_fwd(st, gr, AB_[l], p + W1C, p + B1, S_[l]);
-        mpn_gemm(st, n, HID, HID, S_[l], HID, nullptr, 0, q + B_W2T, HID, p + B2, gr.deg, 0, nullptr, nullptr, M_[l]);
-        mpn_gemm(st, n, 128, HID, H_[l], HID, M_[l], HID, q + B_WN1T, HID, p + BN1, nullptr, 1, nullptr, Z1_[l], U_[l]);
-        mpn_gemm(st, n, HID, HID, U_[l], HID, nullptr, 0, q + B_WN2T, HID, p + BN2, nullptr, 0, H_[l], nullptr,
-                 H_[l + 1]);
-    }
-    mpn_decode(st, n, H_[LAYERS], w + OFF_WD, w + OFF_BD, out ? out : out_);
+void Mpn::forward(const EcnSample &s, float *loss) {
+    ensure((size_t)s.ne, (size_t)s.np);
+    const int E = s.ne;
+    ecn_gemm(st, E, KIN, s.X, KIN, wb + B_W1T, HID, w + OFF_B1, 1, Z1_, H1_, HID);
+    ecn_gemm(st, E, HID, H1_, HID, wb + B_W2T, HID, w + OFF_B2, 1, Z2_, H2_, HID);
+    // FP32 head; k_ecn_pred adds b3
+    ecn_head_fwd(st, E, H2_, s.X, w + OFF_W3, w + OFF_L, O_);
+    ecn_pred(st, s.np, s.rowp, O_, w + OFF_B3, s.Bv, loss ? s.y : nullptr, pred_, D_, loss);
 }
 
-void Mpn::backward(const MpnGraph &gr, const float *X0, const float *y, float *d_loss) {
-    const int n = gr.n;
-    const size_t nh = (size_t)n * HID;
+void Mpn::backward(const EcnSample &s) {
+    const int E = s.ne;
     HIP_OK(hipMemsetAsync(g, 0, sizeof(float) * P, st));
-    mpn_decode_bwd(st, n, H_[LAYERS], w + OFF_WD, out_, y, dH_, g + OFF_WD, g + OFF_BD, d_loss);
-    for (int l = LAYERS - 1; l >= 0; l--) {
-        const float *p = w + L0 + l * LSIZE;
-        float *gp = g + L0 + l * LSIZE;
-        const __bf16 *q = wb + B_L0 + l * B_LSIZE;
-        // H' = H + U Wn2 + bn2
-        mpn_gemm(st, n, HID, HID, dH_, HID, nullptr, 0, q + B_WN2, HID, nullptr, nullptr, 0, nullptr, nullptr, dU_);
-        mpn_wgrad(st, n, HID, HID, U_[l], HID, nullptr, 0, dH_, HID, HID, gp + WN2);
-        mpn_colsum(st, n, HID, dH_, HID, nullptr, gp + BN2);
-        // U = silu([H|M] Wn1 + bn1)
-        mpn_silu_bwd(st, nh, dU_, Z1_[l], dZ1_);
-        mpn_gemm(st, n, HID, HID, dZ1_, HID, nullptr, 0, q + B_WN1, 128, nullptr, nullptr, 0, nullptr, nullptr, dHM_);
-        mpn_wgrad(st, n, 128, HID, H_[l], HID, M_[l], HID, dZ1_, HID, HID, gp + WN1);
-        mpn_colsum(st, n, HID, dZ1_, HID, nullptr, gp + BN1);
-        // M = S W2 + deg b2   (dM = dHM[:, 64:128])
-        mpn_gemm(st, n, HID, HID, dHM_ + HID, 128, nullptr, 0, q + B_W2, HID, nullptr, nullptr, 0, nullptr, nullptr, dS_);
-        mpn_wgrad(st, n, HID, HID, S_[l], HID, nullptr, 0, dHM_ + HID, 128, HID, gp + W2);
-        mpn_colsum(st, n, HID, dHM_ + HID, 128, gr.deg, gp + B2);
-        // S = sum_j silu(A_i + B_j + e W1c + b1)
-        HIP_OK(hipMemsetAsync(dAB_, 0, sizeof(float) * (size_t)n * 128, st));
-        mpn_edge_bwd(st, gr, AB_[l], p + W1C, p + B1, dS_, dAB_, gp + W1C, gp + B1);
-        // AB = H W1ab
-        mpn_wgrad(st, n, HID, HID, H_[l], HID, nullptr, 0, dAB_, 128, 128, gp + W1AB);
-        // dH_in = dH (residual) + dHM[:, 0:64] + dAB W1ab^T
-        mpn_add(st, n, HID, dH_, dHM_, 128);
-        mpn_gemm(st, n, 128, 128, dAB_, 128, nullptr, 0, q + B_W1, HID, nullptr, nullptr, 0, dH_, nullptr, dT_);
-        std::swap(dH_, dT_);
+    // head: dW3, db3, dL and dZ2 = (dO W3^T) hs'(Z2)
+    static_assert(OFF_B3 == OFF_W3 + HID * NOUT && OFF_L == OFF_B3 + NOUT, "head gradient block is contiguous");
+    ecn_head_bwd(st, E, s.pe, s.Bv, D_, H2_, Z2_, s.X, w + OFF_W3, dZ_, g + OFF_W3, part_);
+    ecn_wgrad(st, E, HID, H1_, HID, dZ_, HID, HID, g + OFF_W2, part_);
+    ecn_colsum(st, E, HID, dZ_, HID, g + OFF_B2, part_);
+    // dH1 = dZ2 W2^T, dZ1 = dH1 hs'(Z1)
+    ecn_gemm(st, E, HID, dZ_, HID, wb + B_W2, HID, nullptr, 0, nullptr, dH_, HID);
+    ecn_act_bwd(st, (size_t)E * HID, dH_, Z1_, dZ_);
+    ecn_wgrad(st, E, KIN, s.X, KIN, dZ_, HID, HID, g + OFF_W1, part_);
+    ecn_colsum(st, E, HID, dZ_, HID, g + OFF_B1, part_);
+}
+
+void Mpn::train_steps(int iters, float lr, uint64_t seed) {
+    if (samples.empty() || iters < 1) return;
+    if (!dloss_) dloss_ = alloc<float>(1);
+    std::mt19937_64 rng(seed);
+    std::uniform_int_distribution<size_t> pick(0, samples.size() - 1);
+    for (int it = 0; it < iters; it++) {
+        // every other step on the newest sample, the rest drawn from all
+        const EcnSample &s = (it % 2 == 0) ? samples.back() : samples[pick(rng)];
+        forward(s, dloss_);
+        backward(s);
+        adam(lr);
     }
-    // H0 = silu(X0 We + be)
-    mpn_silu_bwd(st, nh, dH_, Zenc_, dZe_);
-    mpn_wgrad(st, n, NIN, NIN, X0, NIN, nullptr, 0, dZe_, HID, HID, g + OFF_WE);
-    mpn_colsum(st, n, HID, dZe_, HID, nullptr, g + OFF_BE);
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+void Mpn::drop_oldest() {
+    if (samples.empty()) return;
+    EcnSample &s = samples.front();
+    release(s.X); release(s.Bv); release(s.pe); release(s.rowp); release(s.y);
+    samples.erase(samples.begin());
 }
 
 void Mpn::adam(float lr) {
     adam_t++;
-    mpn_adam(st, P, w, g, m1, m2, lr, adam_t);
+    ecn_adam(st, P, w, g, m1, m2, lr, adam_t);
     refresh_bf16();
 }
 

@@ -588,14 +588,16 @@ void kd_build(hipStream_t st, const double *pts, int n, int dim, KdDev &kd, void
 // ---------------------------------------------------------------------------
 // Fixed-radius search.
 
-// Each lane of the search builds and sorts its list in FR_SLOTS LDS slots
+// Each lane of the search builds and sorts its list in SLOTS LDS slots
 // (slot-major, one column per lane of a one-wavefront block) and writes it
 // out once.  A list that outgrows them moves to the output arrays (entries
-// in visit order, then the later hits) and is sorted there.  A few slots keep
-// enough wavefronts resident for the gathers: most lists on a moving lattice
-// at h = 1.3 spacing hold 6 neighbours.
-constexpr int FR_SLOTS = 12;
-constexpr size_t FR_LDS_BYTES = (size_t)FR_SLOTS * WAVE * (sizeof(int) + sizeof(double));
+// in visit order, then the later hits) and is sorted there, one global
+// round trip per step of the insertion sort, which stalls the whole
+// wavefront: the launch picks the smallest of 12, 16, 24 or 32 slots that
+// holds the longest list of the previous search (fewer slots keep more
+// wavefronts resident for the gathers; a lattice at h = 1.3 spacing holds 6
+// neighbours per list, a flowing one 9 on average and up to ~20).
+constexpr int FR_SLOT_CHOICES[4] = {12, 16, 24, 32};
 
 struct KdView {
     const int *parent, *depth, *cd, *is_hi, *leaf_of;
@@ -652,124 +654,143 @@ __device__ __forceinline__ double edge_dij(double mi, double ri, double mj, doub
     return -2.0 * (mi * mj) / (mi + mj) * (ri + rj) / (ri * rj) * rr2 * wfd / (rr2 + 0.01 * h * h);
 }
 
+template <int SLOTS>
 __global__ void __launch_bounds__(WAVE)
 k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts, const int *__restrict__ sorted,
             const int *__restrict__ cell_start, int n, int dim, double h, Grid g, KdView kd, int W,
             int *__restrict__ out_idx, double *__restrict__ out_r2, int *__restrict__ deg, DevErr *err,
             const int *__restrict__ order, FinishArgs fin, int has_fin) {
-    __shared__ int l_idx[FR_SLOTS * WAVE];
-    __shared__ double l_r2[FR_SLOTS * WAVE];
+    __shared__ int l_idx[SLOTS * WAVE];
+    __shared__ double l_r2[SLOTS * WAVE];
     const int tl = (int)threadIdx.x;
     const int chunk = xcd_chunk((m + WAVE - 1) / WAVE);
     if (chunk < 0) return;
     int t = chunk * WAVE + tl;
-    if (t >= m) return;
-    int qi = order ? order[t] : t;
-    auto LI = [&](int k) -> int & { return l_idx[k * WAVE + tl]; };
-    auto LD = [&](int k) -> double & { return l_r2[k * WAVE + tl]; };
-    auto GI = [&](int k) -> int & { return out_idx[(size_t)k * m + qi]; };
-    auto GD = [&](int k) -> double & { return out_r2[(size_t)k * m + qi]; };
-    const int L = W < FR_SLOTS ? W : FR_SLOTS;   // LDS slots in use
-    double qp[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < dim; d++) qp[d] = q[(size_t)d * m + qi];
-    int c[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    for (int d = 0; d < 3; d++) {
-        if (d < dim) {
-            c[d] = cell_coord(qp[d], g.origin[d], g.inv_cell, g.dims[d]);
-            lo[d] = max(c[d] - 1, 0);
-            hi[d] = min(c[d] + 1, g.dims[d] - 1);
-        }
-    }
-    const double sq = h * h;
+    // lanes past the end stay for the wave's reduction of the degree
+    const bool live = t < m;
+    int qi = live ? (order ? order[t] : t) : 0;
     int cnt = 0;
-    for (int cz = lo[2]; cz <= hi[2]; cz++)
-        for (int cy = lo[1]; cy <= hi[1]; cy++) {
-            int row = (cz * g.dims[1] + cy) * g.dims[0];
-            int pb = cell_start[row + lo[0]], pe = cell_start[row + hi[0] + 1];
-            for (int p = pb; p < pe; p++) {
-                const double2 *sp = (const double2 *)(spts + 4 * (size_t)p);
-                const double2 pxy = sp[0];
-                double dist = 0.0;
-                double t0 = qp[0] - pxy.x;
-                dist = dist + t0 * t0;
-                if (dim > 1) { double t1 = qp[1] - pxy.y; dist = dist + t1 * t1; }
-                if (dim > 2) { double t2 = qp[2] - sp[1].x; dist = dist + t2 * t2; }
-                if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) {
-                    const int j = sorted[p];
-                    if (cnt < L) {
-                        LI(cnt) = j;
-                        LD(cnt) = dist;
-                    } else {
-                        if (cnt == L)   // the list moves to the output arrays
-                            for (int k = 0; k < L; k++) { GI(k) = LI(k); GD(k) = LD(k); }
-                        if (cnt < W) { GI(cnt) = j; GD(cnt) = dist; }
+    if (live) {
+        auto LI = [&](int k) -> int & { return l_idx[k * WAVE + tl]; };
+        auto LD = [&](int k) -> double & { return l_r2[k * WAVE + tl]; };
+        auto GI = [&](int k) -> int & { return out_idx[(size_t)k * m + qi]; };
+        auto GD = [&](int k) -> double & { return out_r2[(size_t)k * m + qi]; };
+        const int L = W < SLOTS ? W : SLOTS;   // LDS slots in use
+        double qp[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; d++) qp[d] = q[(size_t)d * m + qi];
+        int c[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < 3; d++) {
+            if (d < dim) {
+                c[d] = cell_coord(qp[d], g.origin[d], g.inv_cell, g.dims[d]);
+                lo[d] = max(c[d] - 1, 0);
+                hi[d] = min(c[d] + 1, g.dims[d] - 1);
+            }
+        }
+        const double sq = h * h;
+        for (int cz = lo[2]; cz <= hi[2]; cz++)
+            for (int cy = lo[1]; cy <= hi[1]; cy++) {
+                int row = (cz * g.dims[1] + cy) * g.dims[0];
+                int pb = cell_start[row + lo[0]], pe = cell_start[row + hi[0] + 1];
+                for (int p = pb; p < pe; p++) {
+                    const double2 *sp = (const double2 *)(spts + 4 * (size_t)p);
+                    const double2 pxy = sp[0];
+                    double dist = 0.0;
+                    double t0 = qp[0] - pxy.x;
+                    dist = dist + t0 * t0;
+                    if (dim > 1) { double t1 = qp[1] - pxy.y; dist = dist + t1 * t1; }
+                    if (dim > 2) { double t2 = qp[2] - sp[1].x; dist = dist + t2 * t2; }
+                    if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) {
+                        const int j = sorted[p];
+                        if (cnt < L) {
+                            LI(cnt) = j;
+                            LD(cnt) = dist;
+                        } else {
+                            if (cnt == L)   // the list moves to the output arrays
+                                for (int k = 0; k < L; k++) { GI(k) = LI(k); GD(k) = LD(k); }
+                            if (cnt < W) { GI(cnt) = j; GD(cnt) = dist; }
+                        }
+                        cnt++;
                     }
-                    cnt++;
                 }
             }
-        }
-    if (cnt > W) {
-        atomicExch(&err->overflow, 1);
-        deg[qi] = 0;
-    } else {
-        deg[qi] = cnt;
-        if (cnt <= L) {
-            sort_list(LI, LD, cnt, kd, qp);
-            // with the finish, r2 is only the spill area of long lists
-            for (int k = 0; k < cnt; k++) {
-                GI(k) = LI(k);
-                if (!has_fin) GD(k) = LD(k);
-            }
+        if (cnt > W) {
+            atomicExch(&err->overflow, 1);
+            deg[qi] = 0;
         } else {
-            sort_list(GI, GD, cnt, kd, qp);
-        }
-        if (has_fin) {
-            // r, D_ij, the neighbour's type and the voxel-major row, in list order
-            const double mi = fin.mass[qi], ri = fin.rho[qi];
-            for (int k = 0; k < cnt; k++) {
-                const int j = cnt <= L ? LI(k) : GI(k);
-                const double rr2 = cnt <= L ? LD(k) : GD(k);
-                const double rr = sqrt(rr2);
-                double mj, rj;
-                int tj;
-                if (fin.rec) {
-                    const PRec *pj = fin.rec + j;
-                    const double2 rm = *(const double2 *)&pj->rho;
-                    rj = rm.x;
-                    mj = rm.y;
-                    tj = pj->type;
-                } else {
-                    mj = fin.mass[j];
-                    rj = fin.rho[j];
-                    tj = fin.type[j];
+            deg[qi] = cnt;
+            if (cnt <= L) {
+                sort_list(LI, LD, cnt, kd, qp);
+                // with the finish, r2 is only the spill area of long lists
+                for (int k = 0; k < cnt; k++) {
+                    GI(k) = LI(k);
+                    if (!has_fin) GD(k) = LD(k);
                 }
-                const double D = edge_dij(mi, ri, mj, rj, rr2, rr, h);
-                if (D != D) {
-                    atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
-                    atomicMin(&err->particle, qi);
+            } else {
+                sort_list(GI, GD, cnt, kd, qp);
+            }
+            if (has_fin) {
+                // r, D_ij, the neighbour's type and the voxel-major row, in list order
+                const double mi = fin.mass[qi], ri = fin.rho[qi];
+                for (int k = 0; k < cnt; k++) {
+                    const int j = cnt <= L ? LI(k) : GI(k);
+                    const double rr2 = cnt <= L ? LD(k) : GD(k);
+                    const double rr = sqrt(rr2);
+                    double mj, rj;
+                    int tj;
+                    if (fin.rec) {
+                        const PRec *pj = fin.rec + j;
+                        const double2 rm = *(const double2 *)&pj->rho;
+                        rj = rm.x;
+                        mj = rm.y;
+                        tj = pj->type;
+                    } else {
+                        mj = fin.mass[j];
+                        rj = fin.rho[j];
+                        tj = fin.type[j];
+                    }
+                    const double D = edge_dij(mi, ri, mj, rj, rr2, rr, h);
+                    if (D != D) {
+                        atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
+                        atomicMin(&err->particle, qi);
+                    }
+                    const size_t e = (size_t)k * m + qi;
+                    fin.r[e] = rr;
+                    fin.Dij[e] = D;
+                    fin.ntype[e] = tj;
+                    fin.erow[(size_t)qi * W + k] = EdgeRec{j, tj, D};
                 }
-                const size_t e = (size_t)k * m + qi;
-                fin.r[e] = rr;
-                fin.Dij[e] = D;
-                fin.ntype[e] = tj;
-                fin.erow[(size_t)qi * W + k] = EdgeRec{j, tj, D};
             }
         }
     }
-    atomicMax(&err->max_deg, cnt);
+    // one atomic per wavefront for the longest list
+    int mx = cnt;
+    for (int o = WAVE / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, WAVE));
+    if (tl == 0) atomicMax(&err->max_deg, mx);
 }
 
 void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted, const int *cell_start,
                int n, int dim, double h, const Grid &g, const KdDev *kd, int W, int *out_idx, double *out_r2,
-               int *deg, DevErr *err, const int *order, const FinishArgs *fin) {
+               int *deg, DevErr *err, const int *order, const FinishArgs *fin, int slots_hint) {
     KdView v{};
     if (kd && kd->valid) {
         v.parent = kd->parent; v.depth = kd->depth; v.cd = kd->cd; v.is_hi = kd->is_hi;
         v.leaf_of = kd->leaf_of; v.cv = kd->cv; v.valid = 1;
     }
     const FinishArgs f = fin ? *fin : FinishArgs{};
-    hipLaunchKernelGGL(k_fr_search, xcd_grid((m + WAVE - 1) / WAVE), dim3(WAVE), 0, s, q, m, spts, sorted, cell_start,
-                       n, dim, h, g, v, W, out_idx, out_r2, deg, err, order, f, fin ? 1 : 0);
+    const dim3 grid = xcd_grid((m + WAVE - 1) / WAVE);
+    const int hf = fin ? 1 : 0;
+    if (slots_hint <= FR_SLOT_CHOICES[0])
+        hipLaunchKernelGGL(k_fr_search<12>, grid, dim3(WAVE), 0, s, q, m, spts, sorted, cell_start, n, dim, h, g, v, W,
+                           out_idx, out_r2, deg, err, order, f, hf);
+    else if (slots_hint <= FR_SLOT_CHOICES[1])
+        hipLaunchKernelGGL(k_fr_search<16>, grid, dim3(WAVE), 0, s, q, m, spts, sorted, cell_start, n, dim, h, g, v, W,
+                           out_idx, out_r2, deg, err, order, f, hf);
+    else if (slots_hint <= FR_SLOT_CHOICES[2])
+        hipLaunchKernelGGL(k_fr_search<24>, grid, dim3(WAVE), 0, s, q, m, spts, sorted, cell_start, n, dim, h, g, v, W,
+                           out_idx, out_r2, deg, err, order, f, hf);
+    else
+        hipLaunchKernelGGL(k_fr_search<32>, grid, dim3(WAVE), 0, s, q, m, spts, sorted, cell_start, n, dim, h, g, v, W,
+                           out_idx, out_r2, deg, err, order, f, hf);
 }
 
 }  // namespace ssa

@@ -260,34 +260,89 @@ def test_bad_descriptor_is_rejected_before_gpu():
 
 def test_mpn_oracle_backward_matches_finite_differences():
     """The surrogate's CPU oracle: its analytic gradient equals central
-    differences of its FP64 loss on a random graph (every parameter block
-    sampled)."""
+    differences of its FP64 loss on a random particle graph (three
+    parameters from each block)."""
     from oracle import mpn_oracle as mo
-    x, X0, deg, nidx, inv_h, y = mo.random_graph(n=40, seed=2, h=0.3)
+    g = mo.random_graph(n=100, seed=2, h=0.3)
+    sc = mo.default_scales(g)
+    sel = np.nonzero(g["solid"] == 0)[0]
+    y = np.random.default_rng(0).normal(0, 1, (len(sel), 3))
     w = mo.init_params(5)
-    _, g = mo.loss_grad(w, x, X0, deg, nidx, inv_h, y)
+    _, gr = mo.loss_grad(w, g, sc, sel, y)
     rng = np.random.default_rng(1)
-    # a few indices from each block
-    sizes = [mo.NIN * mo.HID, mo.HID] + [int(np.prod(s)) for _, s in mo._L] * mo.LAYERS + [mo.HID * 3, 3]
-    offs = np.concatenate([[0], np.cumsum(sizes)])
-    idx = []
+    offs = np.cumsum([0] + [int(np.prod(s)) for _, s in mo._SHAPES])
     for a, b in zip(offs[:-1], offs[1:]):
-        cand = [k for k in range(a, b) if abs(g[k]) > 1e-6] or [a]
-        idx += list(rng.choice(cand, size=min(2, len(cand)), replace=False))
-    for k in idx:
-        e = 1e-6 * max(1.0, abs(w[k]))
-        wp, wm = w.copy(), w.copy()
-        wp[k] += e
-        wm[k] -= e
-        fd = (mo.loss_grad(wp, x, X0, deg, nidx, inv_h, y)[0] - mo.loss_grad(wm, x, X0, deg, nidx, inv_h, y)[0]) / (2 * e)
-        assert abs(fd - g[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, g[k])
+        cand = [k for k in range(a, b) if abs(gr[k]) > 1e-6] or [a]
+        for k in rng.choice(cand, size=min(3, len(cand)), replace=False):
+            e = 1e-6 * max(1.0, abs(w[k]))
+            wp, wm = w.copy(), w.copy()
+            wp[k] += e
+            wm[k] -= e
+            fd = (mo.loss_grad(wp, g, sc, sel, y)[0] - mo.loss_grad(wm, g, sc, sel, y)[0]) / (2 * e)
+            assert abs(fd - gr[k]) <= 1e-4 * max(abs(fd), 1e-3), (k, fd, gr[k])
+
+
+def test_mpn_oracle_pair_terms_span_the_sdpd_force():
+    """The network's output space holds the SDPD pair acceleration.  The
+    pair loop's terms, restated as coefficients of dx, dv, v_i and v_j
+    (mpn_oracle.sdpd_pair_terms), sum to the FP64 SDPD oracle's own F - g on
+    a moving 2D dam break after 40 steps (densities off rho0, velocities,
+    transport velocities, walls); and with those coefficients divided by the
+    envelope as the network's outputs, the oracle's message assembly
+    (features(): basis vectors dx/h, dv/v_s, v_i/v_s, v_j/v_s times the
+    envelope) gives the same accelerations.  A wrong basis vector, sign or
+    envelope breaks one of the two sums."""
+    from oracle import mpn_oracle as mo
+    from oracle import oracle_py
+    from spatialpy_amd import workloads
+    sa = workloads.dam_break_2d(nx=12, ny=12, spacing=0.025, n_mol=20, nt=60, out_every=60, dt=2e-4)
+    o = oracle_py.Oracle(sa)
+    o.reset(seed=3)
+    o.advance(40)
+    st = o.state()
+    rp, idx, dist, dwdr, _ = o.neighbors()
+    F, _, _ = o.eval_forces()
+    gvec = np.asarray(sa.gravity)
+    ref = F - gvec
+    I, J, co = mo.sdpd_pair_terms(st["x"], st["v"], st["vt"], st["rho"], sa.mass, st["nu"], rp, idx, dist, dwdr,
+                                  sa.h, sa.P0, sa.rho0, sa.dimension)
+    x, v = st["x"], st["v"]
+    terms = co[:, :1] * (x[I] - x[J]) + co[:, 1:2] * (v[I] - v[J]) + co[:, 2:3] * v[I] + co[:, 3:4] * v[J]
+    terms[:, sa.dimension:] = 0.0
+    acc = np.zeros_like(ref)
+    np.add.at(acc, I, terms)
+    scale = np.abs(ref[:, :sa.dimension]).max()
+    assert scale > 0 and np.abs(ref[:, sa.dimension:]).max() == 0.0
+    assert np.abs(acc - ref).max() <= 1e-10 * scale, np.abs(acc - ref).max() / scale
+    # the network's assembly with the exact coefficients
+    deg = np.diff(rp)
+    W = int(deg.max())
+    nidx = np.zeros((W, sa.n), np.int32)
+    nr = np.zeros((W, sa.n))
+    for k in range(W):
+        s_ = np.nonzero(deg > k)[0]
+        nidx[k, s_] = idx[rp[s_] + k]
+        nr[k, s_] = dist[rp[s_] + k]
+    g = {"h": sa.h, "x": x, "v": v, "vt": st["vt"], "rho": st["rho"], "mass": sa.mass, "nu": st["nu"],
+         "solid": sa.fixed, "deg": deg, "nidx": nidx, "nr": nr}
+    sc = np.array([sa.rho0, np.sqrt((v * v).sum(1).mean()), 0.01, np.abs(sa.nu).mean(), scale, 1e-3])
+    P_, I2, J2, RD = mo.edges(g, np.arange(sa.n))
+    X, B, u = mo.features(g, sc, I2, J2, RD)
+    valid = u > 0
+    assert np.array_equal(np.stack([I2[valid], J2[valid]]), np.stack([I, J]))
+    o_ = np.zeros((len(I2), 4))
+    o_[valid] = co * np.array([sa.h, sc[1], sc[1], sc[1]]) / (u[valid, None] * sc[4])
+    pred = np.zeros((sa.n, 3))
+    np.add.at(pred, P_, u[:, None] * np.einsum("ek,ekc->ec", o_, B))
+    # basis and envelope are rounded to FP32 as the GPU holds them
+    assert np.abs(pred * sc[4] - ref).max() <= 1e-5 * scale, np.abs(pred * sc[4] - ref).max() / scale
 
 
 def test_mpn_parameter_count_matches_library():
     from oracle import mpn_oracle as mo
     from spatialpy_amd import _abi
     lib = _abi.load_library()
-    assert int(lib.ssasdpd_mpn_num_params()) == mo.P == 77571
+    assert int(lib.ssasdpd_mpn_num_params()) == mo.P == 6596
 
 
 # ---------------------------------------------------------------------------
