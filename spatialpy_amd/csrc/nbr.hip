@@ -687,32 +687,54 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
             }
         }
         const double sq = h * h;
+        // the candidate rows' bounds, all loaded before the first is used
+        int rb[9], re[9], nrow = 0;
         for (int cz = lo[2]; cz <= hi[2]; cz++)
             for (int cy = lo[1]; cy <= hi[1]; cy++) {
-                int row = (cz * g.dims[1] + cy) * g.dims[0];
-                int pb = cell_start[row + lo[0]], pe = cell_start[row + hi[0] + 1];
-                for (int p = pb; p < pe; p++) {
+                const int row = (cz * g.dims[1] + cy) * g.dims[0];
+                rb[nrow] = cell_start[row + lo[0]];
+                re[nrow] = cell_start[row + hi[0] + 1];
+                nrow++;
+            }
+        auto accept = [&](int j, double dist) {
+            if (cnt < L) {
+                LI(cnt) = j;
+                LD(cnt) = dist;
+            } else {
+                if (cnt == L)   // the list moves to the output arrays
+                    for (int k = 0; k < L; k++) { GI(k) = LI(k); GD(k) = LD(k); }
+                if (cnt < W) { GI(cnt) = j; GD(cnt) = dist; }
+            }
+            cnt++;
+        };
+        // candidates in visit order, fetched CB at a time (positions and
+        // tree indices together) so one round trip serves CB of them
+        constexpr int CB = 4;
+        for (int r = 0; r < nrow; r++) {
+            const int pe = re[r];
+            for (int p0 = rb[r]; p0 < pe; p0 += CB) {
+                double2 a[CB], b[CB];
+                int jj[CB];
+#pragma unroll
+                for (int u = 0; u < CB; u++) {
+                    const int p = p0 + u < pe ? p0 + u : p0;
                     const double2 *sp = (const double2 *)(spts + 4 * (size_t)p);
-                    const double2 pxy = sp[0];
+                    a[u] = sp[0];
+                    b[u] = sp[1];
+                    jj[u] = sorted[p];
+                }
+#pragma unroll
+                for (int u = 0; u < CB; u++) {
+                    if (p0 + u >= pe) break;
                     double dist = 0.0;
-                    double t0 = qp[0] - pxy.x;
+                    double t0 = qp[0] - a[u].x;
                     dist = dist + t0 * t0;
-                    if (dim > 1) { double t1 = qp[1] - pxy.y; dist = dist + t1 * t1; }
-                    if (dim > 2) { double t2 = qp[2] - sp[1].x; dist = dist + t2 * t2; }
-                    if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) {
-                        const int j = sorted[p];
-                        if (cnt < L) {
-                            LI(cnt) = j;
-                            LD(cnt) = dist;
-                        } else {
-                            if (cnt == L)   // the list moves to the output arrays
-                                for (int k = 0; k < L; k++) { GI(k) = LI(k); GD(k) = LD(k); }
-                            if (cnt < W) { GI(cnt) = j; GD(cnt) = dist; }
-                        }
-                        cnt++;
-                    }
+                    if (dim > 1) { double t1 = qp[1] - a[u].y; dist = dist + t1 * t1; }
+                    if (dim > 2) { double t2 = qp[2] - b[u].x; dist = dist + t2 * t2; }
+                    if (dist <= sq && dist != 0.0 && sqrt(dist) <= h) accept(jj[u], dist);
                 }
             }
+        }
         if (cnt > W) {
             atomicExch(&err->overflow, 1);
             deg[qi] = 0;

@@ -47,9 +47,8 @@ __device__ __forceinline__ void apply_bcs(const SdpdArgs &a, int i) {
 __device__ __forceinline__ bool bad(double x) { return isnan(x) || !isfinite(x); }
 
 // Particle i's gather record from the arrays (eight 16-byte stores).
-__device__ __forceinline__ void store_rec(const SdpdArgs &a, int i) {
+__device__ __forceinline__ void store_rec_to(const SdpdArgs &a, int i, double2 *p) {
     const size_t n = a.n;
-    double2 *p = (double2 *)(a.rec + i);
     p[0] = make_double2(a.x[i], a.x[n + i]);
     p[1] = make_double2(a.x[2 * n + i], a.v[i]);
     p[2] = make_double2(a.v[n + i], a.v[2 * n + i]);
@@ -59,6 +58,7 @@ __device__ __forceinline__ void store_rec(const SdpdArgs &a, int i) {
     ((int4 *)p)[6] = make_int4(a.type[i], a.solid[i], 0, 0);
     p[7] = make_double2(0.0, 0.0);
 }
+__device__ __forceinline__ void store_rec(const SdpdArgs &a, int i) { store_rec_to(a, i, (double2 *)(a.rec + i)); }
 
 // Neighbour j's record: x, v, vt, nu, rho, mass (six 16-byte loads of one
 // 128-byte line).
@@ -81,11 +81,7 @@ __global__ void k_pack(SdpdArgs a) {
     if (i < a.n) store_rec(a, i);
 }
 
-__global__ void k_predict(SdpdArgs a) {
-    const int chunk = xcd_chunk((a.n + 255) / 256);
-    if (chunk < 0) return;
-    int i = chunk * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+__device__ __forceinline__ void predict_one(const SdpdArgs &a, int i, double2 *rec_out) {
     const size_t n = a.n;
     bool nan = bad(a.rho[i]);
     for (int k = 0; k < 3; k++) nan = nan || bad(a.x[k * n + i]) || bad(a.v[k * n + i]);
@@ -113,7 +109,23 @@ __global__ void k_predict(SdpdArgs a) {
     a.Frho[i] = 0.0;
     for (int s = 0; s < a.Sc; s++) a.Q[(size_t)i * a.Sc + s] = 0.0;
     a.old_rho[i] = a.rho[i];
-    store_rec(a, i);
+    store_rec_to(a, i, rec_out);
+}
+
+// The gather records are staged in LDS and written as one contiguous range
+// per block (a lane storing its own 128-byte record writes 16 bytes to each
+// of 64 lines per instruction).
+__global__ void __launch_bounds__(256) k_predict(SdpdArgs a) {
+    __shared__ double2 srec[256 * 8];
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    const int base = chunk * 256;
+    const int i = base + threadIdx.x;
+    if (i < a.n) predict_one(a, i, srec + 8 * threadIdx.x);
+    __syncthreads();
+    const int nrec = min(256, a.n - base);
+    double2 *dst = (double2 *)(a.rec + base);
+    for (int q = threadIdx.x; q < nrec * 8; q += 256) dst[q] = srec[q];
 }
 
 // generated det<name> functions (solver.py:160-190), called with
