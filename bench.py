@@ -161,9 +161,11 @@ def surrogate_leg(args, sa, E, plain_ms_per_step, plain_pair_ms):
     rec_s = time.perf_counter() - t
     s.stop_recording(eng)
     n_samples = s.num_samples()
+    _progress(f"surrogate: {n_samples} samples recorded in {rec_s:.1f} s")
     t = time.perf_counter()
     loss = s.fit(args.surrogate_epochs, lr=3e-3, lr_end=3e-5)
     train_s = time.perf_counter() - t
+    _progress(f"surrogate: {args.surrogate_epochs} epochs in {train_s:.1f} s")
     train_err = s.relative_error()
     # the run with the network attached; the same window timed
     eng.reset(seed=12345)
@@ -178,6 +180,7 @@ def surrogate_leg(args, sa, E, plain_ms_per_step, plain_pair_ms):
     st_win = Surrogate.stats(eng)
     eng.advance(CYL_RUN_STEPS - first - K)
     st = Surrogate.stats(eng)
+    xs = eng.state()["x"]
     infer_ms = s.time_engine(eng, reps=5)
     s.close()
     eng.close()
@@ -190,6 +193,8 @@ def surrogate_leg(args, sa, E, plain_ms_per_step, plain_pair_ms):
             "checks": st["checks"], "failed_checks": st["failed_checks"],
             "max_passing_check_err": round(st["max_err"], 5), "last_check_err": round(st["last_err"], 5),
             "nn_steps": st["nn_steps"], "nn_share": round(st["nn_steps"] / CYL_RUN_STEPS, 4),
+            "capped_predictions": st["capped"],
+            "final_position_range": [round(float(xs.min()), 4), round(float(xs.max()), 4)],
             "window_ms_per_step": round(1e3 * win_s / K, 4), "window_nn_steps": st_win["nn_steps"],
             "plain_window_ms_per_step": round(plain_ms_per_step, 4),
             "speed_vs_plain_step": round(plain_ms_per_step / (1e3 * win_s / K), 4),
@@ -482,6 +487,10 @@ def bench_dambreak(args):
     print(json.dumps(out))
 
 
+def _progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def bench_cylinder(args):
     """The 1M-particle 3D SDPD + sSSA cylinder: `settle` untimed steps, W
     warm-up steps, K timed steps, then the rest of the 2,000-step run."""
@@ -506,6 +515,7 @@ def bench_cylinder(args):
     eng.advance(settle)
     eng.advance(W)
     pre_s = time.perf_counter() - t_run
+    _progress(f"cylinder: {n} particles, steps 0..{first - 1} in {pre_s:.2f} s")
     # untimed: the state at the window's first step (the CPU sample starts
     # from it) and the neighbour count per particle there
     state = None if args.no_cpu_baseline else eng.state()
@@ -520,11 +530,13 @@ def bench_cylinder(args):
     value = n * K / sec
     events = (st1.reactions + st1.diffusions) - (st0.reactions + st0.diffusions)
     dph = {k: (ph1[k][0] - ph0[k][0], ph1[k][1] - ph0[k][1]) for k in ph1}
+    _progress(f"timed steps {first}..{first + K - 1}: {1e3 * sec / K:.4f} ms/step")
     full = None
     if not args.no_full_run:
         t2 = time.perf_counter()
         eng.advance(CYL_RUN_STEPS - (first + K))
         rest_s = time.perf_counter() - t2
+        _progress(f"rest of the run: {rest_s:.2f} s")
         st2 = eng.stats()
         run_s = pre_s + sec + rest_s
         full = {"steps": CYL_RUN_STEPS, "seconds": round(run_s, 3),
@@ -578,11 +590,17 @@ def bench_cylinder(args):
     if full is not None:
         out["full_run"] = full
     eng.close()
-    if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(args, sa, E, 1e3 * sec / K, dph["forces"][0] / K)
     if not args.no_cpu_baseline:
+        _progress(f"cpu baseline: {cpu_steps} reference steps from step {first}")
         out["cpu_baseline"] = cpu_baseline(sa, state, first, cpu_steps)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    if not args.no_surrogate:
+        from spatialpy_amd.core.errors import SimulationError
+        try:
+            out["surrogate"] = surrogate_leg(args, sa, E, 1e3 * sec / K, dph["forces"][0] / K)
+        except SimulationError as ex:
+            # the plain line stands; the leg's failure is reported with it
+            out["surrogate"] = {"error": str(ex)}
         if "ssa_events_per_s" in out["cpu_baseline"]:
             out["ssa_speedup_vs_cpu"] = out["ssa_events_per_s"] / out["cpu_baseline"]["ssa_events_per_s"]
     print(json.dumps(out))

@@ -99,7 +99,19 @@ enum ssasdpd_flags {
     /* Pair-force terms of pairwiseForce (model.cpp:39-191) in FP32 from
      * FP64-formed dx, dv, summed in FP64; integration stays FP64.  Off: the
      * reference's FP64 arithmetic, operation for operation. */
-    SSASDPD_FLAG_FP32_FORCES = 1u << 4
+    SSASDPD_FLAG_FP32_FORCES = 1u << 4,
+    /* Keep particles in the caller's numbering in device memory.  By default
+     * a moving domain in the seeded mode (not replay, not with
+     * SSASDPD_FLAG_ANN_ORDER_MOVING or an event trace) renumbers its
+     * particles into cell order every 20 steps, so that neighbours in space
+     * are neighbours in memory for the search and every gather; outputs,
+     * state and lists are always reported in the caller's numbering.  The
+     * order inside a list among exactly equal distances follows the internal
+     * numbering, so seeded runs with and without renumbering agree in
+     * distribution, not bit for bit.  A seed still reproduces its run, and
+     * a batch trajectory its single run: the new order is cell order with
+     * ties inside a cell taken by the previous index. */
+    SSASDPD_FLAG_NO_RENUMBER = 1u << 5
 };
 
 typedef struct ssasdpd_desc {
@@ -379,6 +391,15 @@ int ssasdpd_set_surrogate_rearm(ssasdpd_engine *e, int32_t on);
  * online refreshes, failed checks, and whether the network is on. */
 int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *checks, double *last_err,
                             double *max_err, uint64_t *refreshes, uint64_t *failed_checks, int32_t *active);
+/* No counterpart in the reference.  Between checks a particle's network
+ * pair acceleration is capped at factor times the RMS plain pair
+ * acceleration over the moving particles of the last check (default 20; no
+ * cap before the first check, or with factor <= 0), so that a particle in a
+ * state the network has not seen cannot run away before the next check. */
+int ssasdpd_set_surrogate_cap(ssasdpd_engine *e, double factor);
+/* Since the last reset or attach: how many predictions were capped, and the
+ * current cap (0: none). */
+int ssasdpd_surrogate_capped(ssasdpd_engine *e, uint64_t *capped, double *cap);
 
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s);
 
@@ -416,6 +437,14 @@ int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx,
 int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m,
                       int32_t dim, double h, int64_t *row_ptr, int32_t *idx,
                       double *r2);
+
+/* No counterpart in the reference (its search is a kd-tree).  The cell grid
+ * the search bins n points with in dimension dim at radius h, for the
+ * bounding box bbox = {xlo, xhi, ylo, yhi, zlo, zhi}: cells per side, their
+ * total and the cell side.  Host arithmetic only (no GPU); the grid never
+ * exceeds max(4n, 2^20) cells, however far apart the points are. */
+int ssasdpd_grid_shape(const double *bbox, int32_t n, int32_t dim, double h, int32_t *dims, int64_t *ncell,
+                       double *cell);
 
 /* ssasdpd_fr_search without device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (tight

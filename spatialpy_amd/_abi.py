@@ -28,6 +28,7 @@ FLAG_ANN_ORDER_MOVING = 1 << 1
 FLAG_NO_RDME = 1 << 2
 FLAG_TRACE = 1 << 3
 FLAG_FP32_FORCES = 1 << 4
+FLAG_NO_RENUMBER = 1 << 5
 
 BC_CHEM, BC_V, BC_NU, BC_RHO = 0, 1, 2, 3
 
@@ -180,6 +181,9 @@ def _declare(lib):
     lib.ssasdpd_mpn_num_samples.argtypes = [P]
     lib.ssasdpd_mpn_num_samples.restype = C.c_int32
     lib.ssasdpd_surrogate_stats.argtypes = [P, _u64p, _u64p, _dp, _dp, _u64p, _u64p, _ip]
+    lib.ssasdpd_surrogate_capped.argtypes = [P, _u64p, _dp]
+    lib.ssasdpd_set_surrogate_cap.argtypes = [P, C.c_double]
+    lib.ssasdpd_grid_shape.argtypes = [_dp, C.c_int32, C.c_int32, C.c_double, _ip, C.POINTER(C.c_int64), _dp]
     lib.ssasdpd_set_surrogate_rearm.argtypes = [P, C.c_int32]
     lib.ssasdpd_set_surrogate_refresh.argtypes = [P, C.c_int32, C.c_float, C.c_int32, C.c_int32]
     for name in ["ssasdpd_create", "ssasdpd_num_snapshots", "ssasdpd_reset", "ssasdpd_advance",
@@ -192,6 +196,7 @@ def _declare(lib):
                  "ssasdpd_mpn_set_params", "ssasdpd_mpn_set_scales", "ssasdpd_mpn_add_sample",
                  "ssasdpd_mpn_add_engine_sample", "ssasdpd_mpn_clear_samples", "ssasdpd_mpn_forward",
                  "ssasdpd_mpn_grad", "ssasdpd_mpn_train", "ssasdpd_set_surrogate", "ssasdpd_surrogate_stats",
+                 "ssasdpd_surrogate_capped", "ssasdpd_set_surrogate_cap", "ssasdpd_grid_shape",
                  "ssasdpd_mpn_record", "ssasdpd_mpn_time_engine", "ssasdpd_mpn_sample_size",
                  "ssasdpd_mpn_predict", "ssasdpd_set_surrogate_refresh", "ssasdpd_set_surrogate_rearm", "ssasdpd_vm_eval", "ssasdpd_set_timeout", "ssasdpd_fr_search_noalloc"]:
         getattr(lib, name).restype = C.c_int

@@ -184,6 +184,30 @@ struct ssasdpd_engine {
     ssasdpd_stats stats{};
     int max_deg_seen = 0;
     int last_max_deg = 0;       // longest list of the last search
+    // Particle renumbering (moving domains, seeded mode): every
+    // RENUMBER_EVERY steps the particles are put in cell order, so that
+    // neighbours in space are neighbours in memory.  perm[p] is the caller's
+    // index of internal particle p (the identity until the first renumber
+    // and after every reset); outputs go through it back to that numbering.
+    static constexpr unsigned RENUMBER_EVERY = 20;
+    bool renumbered = false;
+    int *perm = nullptr, *perm_tmp = nullptr;
+    // host copies, refreshed on first use after a renumber (the run itself
+    // needs neither)
+    std::vector<int> perm_h;
+    bool perm_h_stale = false, movers_stale = false;
+    const std::vector<int> &perm_host() {
+        if (perm_h_stale) {
+            d2h(st, perm_h.data(), perm, n);
+            HIP_OK(hipStreamSynchronize(st));
+            perm_h_stale = false;
+        }
+        return perm_h;
+    }
+    void *pbuf = nullptr, *stage = nullptr;
+    size_t pbuf_cap = 0, stage_cap = 0;
+    std::vector<double> mass_h, dfn_h;
+    std::vector<int> type_h, solid_h;
     // GPU time per phase, HIP events on the engine stream: 0 binning + tree,
     // 1 search + D_ij, 2 SDPD substeps, 3 rates + queue build, 4 SSA loop,
     // 5 force kernel alone
@@ -223,6 +247,11 @@ struct ssasdpd_engine {
 
     void release() {
         dfree(nn_F); dfree(nn_acc);
+        dfree(perm); dfree(perm_tmp);
+        if (pbuf) (void)hipFree(pbuf);
+        if (stage) (void)hipFree(stage);
+        pbuf = stage = nullptr;
+        pbuf_cap = stage_cap = 0;
         dfree(b_xx); dfree(b_rr); dfree(b_sr); dfree(b_sd); dfree(b_qt); dfree(b_tmin); dfree(b_targ);
         dfree(b_qs); dfree(b_seeds);
         dfree(x); dfree(v); dfree(vt); dfree(F); dfree(Fbp); dfree(normal);
@@ -291,9 +320,25 @@ struct ssasdpd_engine {
     Mpn *nn_rec = nullptr;         // record training samples during plain steps
     int nn_rec_every = 0, nn_rec_max = 32768;
     uint64_t nn_rec_seed = 1;
+    // nn_acc: the check's sums [0], [1], its count of moving particles [2]
+    // and the count of capped predictions [3] (uint64)
     double *nn_F = nullptr, *nn_acc = nullptr;
+    // Between checks the network's pair acceleration of a particle is capped
+    // at nn_cap_factor times the RMS plain pair acceleration of the last
+    // check (off until the first check, or with a factor <= 0): a particle
+    // in a state the network has not seen cannot run away before the next
+    // check catches it.
+    double nn_cap_factor = 20.0;
+    double nn_acc_cap = 0.0;
+    void nn_clear_capped() {
+        if (nn_acc) HIP_OK(hipMemsetAsync(nn_acc + 3, 0, sizeof(double), st));
+    }
     int nn_cap = 0;
-    std::vector<int> movers;       // particles that are not fixed
+    std::vector<int> movers;       // particles that are not fixed (internal numbering)
+    const std::vector<int> &moving() {
+        if (movers_stale) rebuild_movers();
+        return movers;
+    }
 
     // the network's view of the current state: the gather records of the
     // predictor (or of eval_forces) and this step's lists
@@ -306,12 +351,13 @@ struct ssasdpd_engine {
         if (nn_cap >= n) return;
         dfree(nn_F); dfree(nn_acc);
         nn_F = dalloc<double>((size_t)n * 3);
-        nn_acc = dalloc<double>(2);
+        nn_acc = dalloc<double>(4);
+        HIP_OK(hipMemsetAsync(nn_acc, 0, sizeof(double) * 4, st));
         nn_cap = n;
     }
     // up to maxp moving particles, a uniform random subset in index order
-    std::vector<int> pick_particles(int maxp, uint64_t seed) const {
-        std::vector<int> sel = movers;
+    std::vector<int> pick_particles(int maxp, uint64_t seed) {
+        std::vector<int> sel = moving();
         if ((int)sel.size() > maxp) {
             std::mt19937_64 rng(seed);
             for (int k = 0; k < maxp; k++) {
@@ -336,9 +382,11 @@ struct ssasdpd_engine {
 
     // the network's accelerations for the current state into Fout (SoA
     // [3][n])
-    void nn_predict(double *Fout) {
+    // capped: the network drives this step (the acceleration cap applies)
+    void nn_predict(double *Fout, bool capped = false) {
         if (!Fout) throw HipError("surrogate output buffer missing", hipErrorInvalidValue, __FILE__, __LINE__);
-        nn->infer(st, nn_graph(), g, Fout);
+        nn->infer(st, nn_graph(), g, Fout, capped ? nn_acc_cap : 0.0,
+                  capped ? (unsigned long long *)(nn_acc + 3) : nullptr);
     }
 
     // Level-1 nodes held in the event loop's LDS (with the levels above:
@@ -436,6 +484,14 @@ struct ssasdpd_engine {
         std::vector<int> ty(d->type, d->type + n), so(d->fixed, d->fixed + n);
         h2d(st, type, ty.data(), n);
         h2d(st, solid, so.data(), n);
+        type_h = ty;
+        solid_h = so;
+        mass_h.assign(d->mass, d->mass + n);
+        perm = dalloc<int>(n);
+        perm_tmp = dalloc<int>(n);
+        perm_h.resize(n);
+        for (int i = 0; i < n; i++) perm_h[i] = i;
+        h2d(st, perm, perm_h.data(), n);
         any_solid = std::any_of(so.begin(), so.end(), [](int v) { return v != 0; });
         movers.clear();
         for (int i = 0; i < n; i++)
@@ -475,6 +531,7 @@ struct ssasdpd_engine {
             n_df = d->n_data_fn;
             dfn = dalloc<double>((size_t)n * n_df);
             h2d(st, dfn, d->data_fn, (size_t)n * n_df);
+            dfn_h.assign(d->data_fn, d->data_fn + (size_t)n * n_df);
         }
         if (R > 0) {
             int nnz = d->jcN[R];
@@ -527,6 +584,18 @@ struct ssasdpd_engine {
         if (ps) HIP_OK(hipStreamSynchronize(ps));
         la_pending = false;
         size_t n3 = 3 * (size_t)n;
+        if (renumbered) {
+            // the constant per-particle arrays back in the caller's numbering
+            h2d(st, mass, mass_h.data(), n);
+            h2d(st, type, type_h.data(), n);
+            h2d(st, solid, solid_h.data(), n);
+            if (n_df > 0) h2d(st, dfn, dfn_h.data(), (size_t)n * n_df);
+            for (int i = 0; i < n; i++) perm_h[i] = i;
+            h2d(st, perm, perm_h.data(), n);
+            renumbered = false;
+            perm_h_stale = false;
+            rebuild_movers();
+        }
         h2d(st, x, x0.data(), n3);
         h2d(st, v, v0.data(), n3);
         HIP_OK(hipMemsetAsync(vt, 0, sizeof(double) * n3, st));
@@ -581,6 +650,8 @@ struct ssasdpd_engine {
         nn_max_err = 0.0;
         nn_refreshes = 0;
         nn_fails = 0;
+        nn_acc_cap = 0.0;
+        nn_clear_capped();
         for (int p = 0; p < NPH; p++) { ph_ms[p] = 0; ph_launch[p] = 0; pending[p][0] = pending[p][1] = false; }
     }
 
@@ -593,6 +664,82 @@ struct ssasdpd_engine {
 
     // Bin the tree points (= current x), and build the kd-tree when list
     // order needs it.
+    void rebuild_movers() {
+        const std::vector<int> &ph = perm_host();
+        movers.clear();
+        for (int p = 0; p < n; p++)
+            if (!solid_h[ph[p]]) movers.push_back(p);
+        movers_stale = false;
+    }
+
+    bool renumber_allowed() const {
+        return static_domain == 0 && rng_kind == 0 && trace_cap == 0 &&
+               !(flags & (SSASDPD_FLAG_ANN_ORDER_MOVING | SSASDPD_FLAG_NO_RENUMBER));
+    }
+    bool renumber_due(unsigned step) const { return step > 0 && step % RENUMBER_EVERY == 0 && renumber_allowed(); }
+
+    static void *grow(void *&buf, size_t &cap, size_t bytes) {
+        if (bytes > cap) {
+            if (buf) HIP_OK(hipFree(buf));
+            HIP_OK(hipMalloc(&buf, bytes));
+            cap = bytes;
+        }
+        return buf;
+    }
+    // an [n][cols] array of esz-byte elements gathered into the new order
+    void permute_inplace(void *arr, int cols, int esz, const int *map) {
+        if (!arr || cols <= 0) return;
+        const size_t bytes = (size_t)n * cols * esz;
+        void *tmp = grow(pbuf, pbuf_cap, bytes);
+        permute_rows(st, n, cols, esz, map, arr, tmp, false);
+        HIP_OK(hipMemcpyAsync(arr, tmp, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    // Put the particles in cell order of their current positions: every
+    // array that carries state from one step to the next, the counts of
+    // n_traj batch trajectories, and the map to the caller's numbering.
+    // Lists, rates and the queue are rebuilt by the next step.
+    void renumber(int n_traj = 0) {
+        bin_tree_points();                        // sorted[p]: the p-th particle in cell order
+        // ties inside a cell by index, so the numbering (which the random
+        // decisions of the chemistry see) is a function of the positions
+        cell_order_by_index(st, cell_start, grid.ncell, sorted);
+        const int *m = sorted;
+        for (double *a : {x, v, vt, F, Fbp, normal})
+            for (int k = 0; k < 3; k++) permute_inplace(a + (size_t)k * n, 1, 8, m);
+        for (double *a : {rho, old_rho, Frho, mass, nu, bvf}) permute_inplace(a, 1, 8, m);
+        permute_inplace(type, 1, 4, m);
+        permute_inplace(solid, 1, 4, m);
+        if (Sc > 0) {
+            permute_inplace(C, Sc, 8, m);
+            permute_inplace(Q, Sc, 8, m);
+        }
+        if (S > 0) permute_inplace(xx, S, 4, m);
+        for (int b = 0; b < n_traj; b++) permute_inplace(b_xx + (size_t)b * n * S, S, 4, m);
+        if (n_df > 0) permute_inplace(dfn, n_df, 8, m);
+        permute_rows(st, n, 1, 4, m, perm, perm_tmp, false);
+        std::swap(perm, perm_tmp);
+        renumbered = true;
+        rec_ready = false;
+        perm_h_stale = movers_stale = true;
+    }
+    // an [n][cols] device array in the caller's numbering (the array itself
+    // while no renumber has happened); the result lives until the next call
+    template <class T> const T *orig(const T *src, int cols) {
+        static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte elements");
+        if (!renumbered || !src || cols <= 0) return src;
+        void *out = grow(stage, stage_cap, (size_t)n * cols * sizeof(T));
+        permute_rows(st, n, cols, (int)sizeof(T), perm, src, out, true);
+        return (const T *)out;
+    }
+    // a component-major [3][n] array in the caller's numbering
+    const double *orig3(const double *src) {
+        if (!renumbered || !src) return src;
+        double *out = (double *)grow(stage, stage_cap, 3 * (size_t)n * sizeof(double));
+        for (int k = 0; k < 3; k++) permute_rows(st, n, 1, 8, perm, src + (size_t)k * n, out + (size_t)k * n, true);
+        return out;
+    }
+    int orig_index(int p) { return (renumbered && p >= 0 && p < n) ? perm_host()[p] : p; }
+
     void bin_tree_points() {
         ph_begin(0);
         HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
@@ -610,7 +757,8 @@ struct ssasdpd_engine {
             cell_start = dalloc<int>(cell_cap + 1);
             cell_fill = dalloc<int>(cell_cap);
         }
-        bin_points(st, xtree, n, dim, grid, cell_of, cell_count, cell_start, cell_fill, sorted, spts, scan_tmp);
+        bin_points(st, xtree, n, dim, grid, cell_of, cell_count, cell_start, cell_fill, sorted, spts, scan_tmp,
+                   &derr->crowded);
         bool want_kd = static_domain || (flags & SSASDPD_FLAG_ANN_ORDER_MOVING);
         if (want_kd && !(static_domain && kd_built)) {
             kd_build(st, xtree, n, dim, kd, kd_work, kd_work_sz);
@@ -692,8 +840,19 @@ struct ssasdpd_engine {
     int check(unsigned step, bool particles = true) {
         DevErr e{};
         if (particles) e = read_err();
+        if (e.code == SSASDPD_ERR_CAPACITY && e.crowded > 0) {
+            stats.bad_particle = -1;
+            char buf[320];
+            snprintf(buf, sizeof buf,
+                     "step %u: %d particles share one cell of the neighbour-search grid (at most %d): the bounding "
+                     "box [%g, %g] x [%g, %g] x [%g, %g] is too wide for %d particles at h = %g (a particle far from "
+                     "the others?)",
+                     step, e.crowded, FR_MAX_CELL, h_bbox[0], h_bbox[1], h_bbox[2], h_bbox[3], h_bbox[4], h_bbox[5],
+                     n, h);
+            return set_error(e.code, buf);
+        }
         if (e.code) {
-            stats.bad_particle = e.particle == INT32_MAX ? -1 : e.particle;
+            stats.bad_particle = e.particle == INT32_MAX ? -1 : orig_index(e.particle);
             char buf[160];
             snprintf(buf, sizeof buf, "step %u: particle %d", step, stats.bad_particle);
             return set_error(e.code, e.code == SSASDPD_ERR_NAN_STATE ? std::string("nan/inf state at ") + buf
@@ -749,7 +908,7 @@ struct ssasdpd_engine {
             sa.skip_F = 1;
             sdpd_forces(st, sa);
             sa.skip_F = 0;
-            nn_predict(F);
+            nn_predict(F, true);
             nn_steps++;
         } else {
             sdpd_forces(st, sa);                   // substep 1
@@ -760,12 +919,13 @@ struct ssasdpd_engine {
                 // network is switched off (for the rest of the run, or until
                 // a check passes again with re-arm)
                 nn_predict(nn_F);
-                HIP_OK(hipMemsetAsync(nn_acc, 0, sizeof(double) * 2, st));
+                HIP_OK(hipMemsetAsync(nn_acc, 0, sizeof(double) * 3, st));
                 ecn_err(st, n, F, nn_F, solid, g, nn_acc);
-                double acc[2];
-                d2h(st, acc, nn_acc, 2);
+                double acc[3];
+                d2h(st, acc, nn_acc, 3);
                 HIP_OK(hipStreamSynchronize(st));
                 nn_last_err = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
+                nn_acc_cap = (nn_cap_factor > 0.0 && acc[2] > 0.0) ? nn_cap_factor * std::sqrt(acc[1] / acc[2]) : 0.0;
                 nn_checks++;
                 nn_active = nn_last_err <= nn_tol;
                 if (nn_active)
@@ -881,6 +1041,8 @@ struct ssasdpd_engine {
     // its particle phase may start now, beside this step's event loop.
     int do_step(bool lookahead = false) {
         const unsigned step = cur;
+        // (a renumber step is never prepared by a look-ahead, see below)
+        if (!la_pending && renumber_due(step)) renumber();
         if (la_pending) {
             HIP_OK(hipStreamWaitEvent(st, la_done, 0));
             la_pending = false;
@@ -898,7 +1060,7 @@ struct ssasdpd_engine {
                 QueueState q;
                 d2h(st, &q, qs, 1);
                 // this step's particle errors, read before a look-ahead can add the next step's
-                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt;
+                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt && !renumber_due(step + 1);
                 DevErr pe{};
                 if (want) d2h(st, &pe, derr, 1);
                 HIP_OK(hipStreamSynchronize(st));
@@ -1005,6 +1167,7 @@ struct ssasdpd_engine {
 
     int batch_step(int B) {
         const unsigned step = cur;
+        if (renumber_due(step)) renumber(B);
         particle_phase(step);
         int rc = check(step);
         if (rc) return rc;
@@ -1063,20 +1226,20 @@ struct ssasdpd_engine {
     void snapshot(ssasdpd_outputs *o, int k) {
         size_t nn = n;
         std::vector<double> tmp;
-        if (o->xx && S > 0) d2h(st, o->xx + (size_t)k * nn * S, xx, nn * S);
-        if (o->C && Sc > 0) d2h(st, o->C + (size_t)k * nn * Sc, C, nn * Sc);
-        if (o->rho) d2h(st, o->rho + (size_t)k * nn, rho, nn);
-        if (o->bvf_phi) d2h(st, o->bvf_phi + (size_t)k * nn, bvf, nn);
-        if (o->nu) d2h(st, o->nu + (size_t)k * nn, nu, nn);
+        if (o->xx && S > 0) d2h(st, o->xx + (size_t)k * nn * S, orig(xx, S), nn * S);
+        if (o->C && Sc > 0) d2h(st, o->C + (size_t)k * nn * Sc, orig(C, Sc), nn * Sc);
+        if (o->rho) d2h(st, o->rho + (size_t)k * nn, orig(rho, 1), nn);
+        if (o->bvf_phi) d2h(st, o->bvf_phi + (size_t)k * nn, orig(bvf, 1), nn);
+        if (o->nu) d2h(st, o->nu + (size_t)k * nn, orig(nu, 1), nn);
         if (o->x) {
             tmp.resize(3 * nn);
-            d2h(st, tmp.data(), x, 3 * nn);
+            d2h(st, tmp.data(), orig3(x), 3 * nn);
             HIP_OK(hipStreamSynchronize(st));
             from_soa3(tmp, o->x + (size_t)k * nn * 3, n);
         }
         if (o->v) {
             tmp.resize(3 * nn);
-            d2h(st, tmp.data(), v, 3 * nn);
+            d2h(st, tmp.data(), orig3(v), 3 * nn);
             HIP_OK(hipStreamSynchronize(st));
             from_soa3(tmp, o->v + (size_t)k * nn * 3, n);
         }
@@ -1473,7 +1636,7 @@ int ssasdpd_run_batch(ssasdpd_engine *e, int32_t n_traj, const uint64_t *seeds, 
         snap_shared(k, e->cur);
         if (counts && e->S > 0) {
             for (int b = 0; b < B; b++)
-                d2h(e->st, counts + ((size_t)b * n_snap + k) * per, e->b_xx + (size_t)b * per, per);
+                d2h(e->st, counts + ((size_t)b * n_snap + k) * per, e->orig(e->b_xx + (size_t)b * per, e->S), per);
             HIP_OK(hipStreamSynchronize(e->st));
         }
     };
@@ -1556,9 +1719,9 @@ int ssasdpd_eval_forces(ssasdpd_engine *e, int32_t fp32, double *F, double *Fbp,
     sdpd_pack(e->st, a);      // gather records of the current state
     sdpd_forces(e->st, a);
     std::vector<double> tmp(n3);
-    if (F) { d2h(e->st, tmp.data(), dF, n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, F, (int)n); }
-    if (Fbp) { d2h(e->st, tmp.data(), dB, n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, Fbp, (int)n); }
-    if (Frho) d2h(e->st, Frho, dR, n);
+    if (F) { d2h(e->st, tmp.data(), e->orig3(dF), n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, F, (int)n); }
+    if (Fbp) { d2h(e->st, tmp.data(), e->orig3(dB), n3); HIP_OK(hipStreamSynchronize(e->st)); from_soa3(tmp, Fbp, (int)n); }
+    if (Frho) d2h(e->st, Frho, e->orig(dR, 1), n);
     HIP_OK(hipStreamSynchronize(e->st));
     dfree(dF); dfree(dB); dfree(dR); dfree(dQ);
     return 0;
@@ -1994,7 +2157,7 @@ int ssasdpd_mpn_add_engine_sample(ssasdpd_mpn *m, ssasdpd_engine *e, int32_t max
     GUARD_BEGIN
     if (!e->lists_built) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: engine has no neighbour lists yet");
     if (max_particles < 1) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: max_particles must be >= 1");
-    if (e->movers.empty()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: the engine has no moving particles");
+    if (e->moving().empty()) return set_error(SSASDPD_ERR_BAD_DESC, "mpn: the engine has no moving particles");
     const size_t n = e->n, n3 = 3 * n;
     // the plain pair-force kernel on the current state is the target
     double *dF = dalloc<double>(n3), *dB = dalloc<double>(n3), *dR = dalloc<double>(n);
@@ -2066,6 +2229,8 @@ int ssasdpd_set_surrogate(ssasdpd_engine *e, ssasdpd_mpn *m, int32_t check_every
     e->nn_max_err = 0.0;
     e->nn_refreshes = 0;
     e->nn_fails = 0;
+    e->nn_acc_cap = 0.0;
+    e->nn_clear_capped();
     return 0;
 }
 
@@ -2097,6 +2262,25 @@ int ssasdpd_surrogate_stats(ssasdpd_engine *e, uint64_t *nn_steps, uint64_t *che
     return 0;
 }
 
+int ssasdpd_set_surrogate_cap(ssasdpd_engine *e, double factor) {
+    e->nn_cap_factor = factor;
+    if (factor <= 0.0) e->nn_acc_cap = 0.0;
+    return 0;
+}
+
+int ssasdpd_surrogate_capped(ssasdpd_engine *e, uint64_t *capped, double *cap) {
+    GUARD_BEGIN
+    uint64_t c = 0;
+    if (e->nn_acc) {
+        d2h(e->st, &c, (const uint64_t *)(e->nn_acc + 3), 1);
+        HIP_OK(hipStreamSynchronize(e->st));
+    }
+    if (capped) *capped = c;
+    if (cap) *cap = e->nn_acc_cap;
+    return 0;
+    GUARD_END
+}
+
 int ssasdpd_stats_get(ssasdpd_engine *e, ssasdpd_stats *s) {
     *s = e->stats;
     return 0;
@@ -2115,23 +2299,30 @@ int ssasdpd_get_state(ssasdpd_engine *e, ssasdpd_state *s) {
     const int n = e->n;
     hipStream_t st = e->st;
     std::vector<double> tmp(3 * (size_t)n);
+    // every array in the caller's numbering
     auto v3 = [&](double *dst, double *src) {
         if (!dst) return;
-        d2h(st, tmp.data(), src, 3 * (size_t)n);
+        d2h(st, tmp.data(), e->orig3(src), 3 * (size_t)n);
         HIP_OK(hipStreamSynchronize(st));
         from_soa3(tmp, dst, n);
     };
     v3(s->x, e->x); v3(s->v, e->v); v3(s->vt, e->vt); v3(s->F, e->F); v3(s->Fbp, e->Fbp);
-    auto v1 = [&](double *dst, double *src, size_t cnt) { if (dst && src) d2h(st, dst, src, cnt); };
-    v1(s->rho, e->rho, n); v1(s->Frho, e->Frho, n); v1(s->bvf_phi, e->bvf, n); v1(s->nu, e->nu, n);
-    v1(s->old_rho, e->old_rho, n);
-    if (e->Sc) { v1(s->C, e->C, (size_t)n * e->Sc); v1(s->Q, e->Q, (size_t)n * e->Sc); }
+    auto v1 = [&](double *dst, double *src, int cols) {
+        if (dst && src) {
+            d2h(st, dst, e->orig(src, cols), (size_t)n * cols);
+            HIP_OK(hipStreamSynchronize(st));
+        }
+    };
+    v1(s->rho, e->rho, 1); v1(s->Frho, e->Frho, 1); v1(s->bvf_phi, e->bvf, 1); v1(s->nu, e->nu, 1);
+    v1(s->old_rho, e->old_rho, 1);
+    if (e->Sc) { v1(s->C, e->C, e->Sc); v1(s->Q, e->Q, e->Sc); }
     if (e->S) {
-        if (s->xx) d2h(st, s->xx, e->xx, (size_t)n * e->S);
-        v1(s->srrate, e->srrate, n); v1(s->sdrate, e->sdrate, n);
-        if (e->R) v1(s->rrate, e->rrate, (size_t)n * e->R);
-        v1(s->Ddiag, e->Ddiag, (size_t)n * e->S);
-        v1(s->next_time, e->qtime, n);
+        if (s->xx) d2h(st, s->xx, e->orig(e->xx, e->S), (size_t)n * e->S);
+        HIP_OK(hipStreamSynchronize(st));
+        v1(s->srrate, e->srrate, 1); v1(s->sdrate, e->sdrate, 1);
+        if (e->R) v1(s->rrate, e->rrate, e->R);
+        v1(s->Ddiag, e->Ddiag, e->S);
+        v1(s->next_time, e->qtime, 1);
     }
     HIP_OK(hipStreamSynchronize(st));
     return 0;
@@ -2142,7 +2333,7 @@ int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx, dou
     GUARD_BEGIN
     const int n = e->n, W = e->W;
     std::vector<int> deg(n);
-    d2h(e->st, deg.data(), e->deg, n);
+    d2h(e->st, deg.data(), e->orig(e->deg, 1), n);
     HIP_OK(hipStreamSynchronize(e->st));
     int64_t off = 0;
     for (int i = 0; i < n; i++) { row_ptr[i] = off; off += deg[i]; }
@@ -2156,21 +2347,40 @@ int ssasdpd_get_neighbors(ssasdpd_engine *e, int64_t *row_ptr, int32_t *idx, dou
     d2h(e->st, er.data(), e->erow, (size_t)W * n);
     HIP_OK(hipStreamSynchronize(e->st));
     const double h = e->h, a = e->alpha;
+    // rows in the caller's numbering: caller's particle i is internal p
+    std::vector<int> inv;
+    if (e->renumbered) {
+        inv.resize(n);
+        const std::vector<int> &ph = e->perm_host();
+        for (int p = 0; p < n; p++) inv[ph[p]] = p;
+    }
     for (int i = 0; i < n; i++)
         for (int k = 0; k < deg[i]; k++) {
-            size_t s = (size_t)k * n + i;
+            const int p = e->renumbered ? inv[i] : i;
+            size_t s = (size_t)k * n + p;
             int64_t o = row_ptr[i] + k;
-            idx[o] = ni[s];
+            idx[o] = e->orig_index(ni[s]);
             double r = nr[s];
             if (dist) dist[o] = r;
             if (dWdr) {
                 double R = r / h;
                 dWdr[o] = a * (-12 * r / (h * h)) * ((1 - R) * (1 - R));   // particle.cpp:178
             }
-            if (Dij) Dij[o] = er[(size_t)i * W + k].D;
+            if (Dij) Dij[o] = er[(size_t)p * W + k].D;
         }
     return 0;
     GUARD_END
+}
+
+int ssasdpd_grid_shape(const double *bbox, int32_t n, int32_t dim, double h, int32_t *dims, int64_t *ncell,
+                       double *cell) {
+    if (!bbox || n < 1 || dim < 1 || dim > 3 || !(h > 0.0)) return set_error(SSASDPD_ERR_BAD_DESC, "grid_shape: bad arguments");
+    Grid g;
+    grid_setup(bbox, n, dim, h, g);
+    for (int d = 0; d < 3 && dims; d++) dims[d] = g.dims[d];
+    if (ncell) *ncell = g.ncell;
+    if (cell) *cell = 1.0 / g.inv_cell;
+    return 0;
 }
 
 int ssasdpd_fr_search(const double *pts, int32_t n, const double *q, int32_t m, int32_t dim, double h,

@@ -57,6 +57,7 @@ struct DevErr {
     unsigned step;
     int max_deg;       // largest neighbour count seen by the last search
     int overflow;      // a list did not fit in the ELL width
+    int crowded;       // a binned cell's count when it passed FR_MAX_CELL, else 0
 };
 
 // Mass-action reaction (same as ssasdpd_reaction, device copy).
@@ -126,9 +127,19 @@ struct Bufs;   // defined in engine.hip
 // ELL rows (idx, r2) of width W.  Returns via err->overflow when W is too
 // small.
 void grid_setup(const double *pts_host_bbox, int n, int dim, double h, Grid &g);
+// Most points one search cell may hold: past it the grid is taken as broken
+// (one point far from the rest coarsens every cell until the bulk of the
+// points shares a few), and the engine's search reports an error instead of
+// scanning them for every query.
+constexpr int FR_MAX_CELL = 4096;
 void bin_points(hipStream_t s, const double *pts, int n, int dim, const Grid &g, int *cell_of,
                 int *cell_count, int *cell_start, int *cell_fill, int *sorted, double *spts,
-                void *scan_tmp);   // spts: sorted points [n][4] (x, y, z, 0)
+                void *scan_tmp,    // spts: sorted points [n][4] (x, y, z, 0)
+                int *crowded = nullptr);   // raised to a cell's count past FR_MAX_CELL
+// Within each cell of a binning, the point indices of sorted[] in ascending
+// order (the scatter's atomics leave them in arrival order).  spts is not
+// reordered: the result is an ordering, not a binning to search.
+void cell_order_by_index(hipStream_t s, const int *cell_start, long long ncell, int *sorted);
 void compute_bbox(hipStream_t s, const double *pts, int n, int dim, double *d_bbox /*[6]*/,
                   unsigned long long *tmp);
 // One edge of a voxel's neighbour row as the event loop reads it: a row is
@@ -277,6 +288,9 @@ void rdme_take_step_multi(hipStream_t s, const RdmeArgs *d_args, int n_models, i
 void vm_eval_batch(hipStream_t s, const uint32_t *code, int start, const double *K, int x_is_double, int n_x,
                    int n_df, int m, const double *xd, const uint32_t *xu, const double *vol, const double *t,
                    const double *df, double *out);
+// Row permutation of an [n][cols] array of esz-byte elements (4 or 8):
+// gather dst[p] = src[map[p]], or scatter dst[map[p]] = src[p] (sdpd.hip).
+void permute_rows(hipStream_t s, int n, int cols, int esz, const int *map, const void *src, void *dst, bool scatter);
 size_t ssa_lds_bytes(int tree_nodes, int S, int R, int T, int nnzN, int nnzG, int n_code = 0, int n_consts = 0);
 
 }  // namespace ssa

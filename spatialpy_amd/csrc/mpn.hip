@@ -195,7 +195,8 @@ constexpr int ECN_PER_WAVE = 16;
 
 __global__ void __launch_bounds__(256) k_ecn_infer(EcnGraph g, EcnScales s, const __bf16 *__restrict__ wb,
                                                    const float *__restrict__ w, double g0, double g1, double g2,
-                                                   double *__restrict__ F) {
+                                                   double *__restrict__ F, double cap,
+                                                   unsigned long long *__restrict__ n_capped) {
     __shared__ __align__(16) __bf16 sw1[Mpn::HID * Mpn::KIN];
     __shared__ __align__(16) __bf16 sw2[Mpn::HID * Mpn::HID];
     __shared__ __align__(16) float sw3f[Mpn::HID * Mpn::NOUT];
@@ -318,9 +319,20 @@ __global__ void __launch_bounds__(256) k_ecn_infer(EcnGraph g, EcnScales s, cons
     if (grp == 0 && live) {
         const bool mv = pi.solid == 0.f;
         const double gg[3] = {g0, g1, g2};
+        double a[3];
+        double a2 = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            F[(size_t)c * g.n + p] = gg[c] + (mv && c < g.dim ? (double)(s.acc * acc[c]) : 0.0);
+        for (int c = 0; c < 3; c++) {
+            a[c] = (mv && c < g.dim) ? (double)(s.acc * acc[c]) : 0.0;
+            a2 += a[c] * a[c];
+        }
+        double f = 1.0;
+        if (cap > 0.0 && a2 > cap * cap) {
+            f = cap / sqrt(a2);
+            if (n_capped) atomicAdd(n_capped, 1ULL);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) F[(size_t)c * g.n + p] = gg[c] + a[c] * f;
     }
 }
 
@@ -639,7 +651,7 @@ __global__ void k_ecn_bf16(const float *__restrict__ w, __bf16 *__restrict__ wb)
 __global__ void k_ecn_err(int n, const double *__restrict__ Fref, const double *__restrict__ Fnn,
                           const int *__restrict__ solid, double g0, double g1, double g2, double *__restrict__ acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double e = 0.0, r = 0.0;
+    double e = 0.0, r = 0.0, c1 = 0.0;
     if (i < n && solid[i] == 0) {
         const double gg[3] = {g0, g1, g2};
         for (int c = 0; c < 3; c++) {
@@ -647,9 +659,18 @@ __global__ void k_ecn_err(int n, const double *__restrict__ Fref, const double *
             e += (a - b) * (a - b);
             r += (a - gg[c]) * (a - gg[c]);
         }
+        c1 = 1.0;
     }
-    for (int o = 32; o > 0; o >>= 1) { e += __shfl_xor(e, o, 64); r += __shfl_xor(r, o, 64); }
-    if ((threadIdx.x & 63) == 0) { atomicAdd(&acc[0], e); atomicAdd(&acc[1], r); }
+    for (int o = 32; o > 0; o >>= 1) {
+        e += __shfl_xor(e, o, 64);
+        r += __shfl_xor(r, o, 64);
+        c1 += __shfl_xor(c1, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&acc[0], e);
+        atomicAdd(&acc[1], r);
+        atomicAdd(&acc[2], c1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -657,9 +678,10 @@ __global__ void k_ecn_err(int n, const double *__restrict__ Fref, const double *
 static unsigned blocks(size_t m, int t) { return (unsigned)((m + t - 1) / t); }
 
 void ecn_infer(hipStream_t s, const EcnGraph &g, const EcnScales &sc, const __bf16 *wb, const float *w,
-               const double *gr, double *F) {
+               const double *gr, double *F, double cap, unsigned long long *n_capped) {
     const int nb = (g.n + 4 * ECN_PER_WAVE - 1) / (4 * ECN_PER_WAVE);
-    hipLaunchKernelGGL(k_ecn_infer, xcd_grid(nb), dim3(256), 0, s, g, sc, wb, w, gr[0], gr[1], gr[2], F);
+    hipLaunchKernelGGL(k_ecn_infer, xcd_grid(nb), dim3(256), 0, s, g, sc, wb, w, gr[0], gr[1], gr[2], F, cap,
+                       n_capped);
 }
 void ecn_gemm(hipStream_t s, int M, int K, const float *A, int lda, const __bf16 *Wt, int N, const float *bias, int act,
               float *Z, float *Y, int ldy) {

@@ -64,7 +64,8 @@ class Mpn {
 
     // F (SoA [3][n]) = g + acc_scale * (predicted pair acceleration) for
     // moving particles, g for solid ones; on stream s.
-    void infer(hipStream_t s, const EcnGraph &gr, const double *gvec, double *F);
+    void infer(hipStream_t s, const EcnGraph &gr, const double *gvec, double *F, double cap = 0.0,
+               unsigned long long *n_capped = nullptr);
     // materialise the edges of the n_sel particles sel_host[] of gr as a new
     // sample; targets from y_host ([n_sel][3], already / acc_scale) or, when
     // null, from F_dev (SoA [3][n] accelerations, gravity gvec) on the device
@@ -93,8 +94,11 @@ class Mpn {
 };
 
 // launchers (mpn.hip)
+// cap > 0: a moving particle's predicted pair acceleration is scaled down to
+// magnitude cap where it is larger, and n_capped (if given) counts those
+// particles
 void ecn_infer(hipStream_t s, const EcnGraph &g, const EcnScales &sc, const __bf16 *wb, const float *w,
-               const double *gr, double *F);
+               const double *gr, double *F, double cap = 0.0, unsigned long long *n_capped = nullptr);
 void ecn_gemm(hipStream_t s, int M, int K, const float *A, int lda, const __bf16 *Wt, int N, const float *bias, int act,
               float *Z, float *Y, int ldy);
 // weight gradients are summed without atomics: per-block partials in
@@ -118,6 +122,8 @@ void ecn_head_bwd(hipStream_t s, int E, const int *pe, const float *Bv, const fl
                   const float *Z2, const float *X, const float *W3, float *dZ2, float *g_head, float *part);
 void ecn_adam(hipStream_t s, int m, float *w, const float *g, float *mo, float *ve, float lr, int t);
 void ecn_bf16(hipStream_t s, const float *w, __bf16 *wb);
+// over moving particles: acc[0] += |Fref - Fnn|^2, acc[1] += |Fref - g|^2,
+// acc[2] += 1
 void ecn_err(hipStream_t s, int n, const double *Fref, const double *Fnn, const int *solid, const double *g,
              double *acc);
 

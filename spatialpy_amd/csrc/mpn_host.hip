@@ -63,10 +63,11 @@ void Mpn::get_params(float *host) {
 
 void Mpn::refresh_bf16() { ecn_bf16(st, w, wb); }
 
-void Mpn::infer(hipStream_t s, const EcnGraph &gr, const double *gvec, double *F) {
+void Mpn::infer(hipStream_t s, const EcnGraph &gr, const double *gvec, double *F, double cap,
+                unsigned long long *n_capped) {
     // the weights were last written on the network's own stream
     HIP_OK(hipStreamSynchronize(st));
-    ecn_infer(s, gr, sc, wb, w, gvec, F);
+    ecn_infer(s, gr, sc, wb, w, gvec, F, cap, n_capped);
 }
 
 void Mpn::clear_samples() {

@@ -173,24 +173,30 @@ void compute_bbox(hipStream_t s, const double *pts, int n, int dim, double *d_bb
 }
 
 // Cell grid with side >= h over the bounding box bb[6] (host values).
+// Counts are formed in double: a point far from the rest (a bounding box of
+// 1e7 cells per side) must coarsen the grid, not wrap an integer product.
+// A non-finite extent (NaN/Inf positions, reported by the state check) gets
+// one cell.
 void grid_setup(const double *bb, int n, int dim, double h, Grid &g) {
+    const double limit = (double)std::max<long long>(4LL * n, 1 << 20);
     double cell = h * 1.01;
     for (;;) {
-        long long tot = 1;
-        for (int d = 0; d < 3; d++) {
-            if (d < dim) {
-                double ext = bb[2 * d + 1] - bb[2 * d];
-                long long c = (long long)std::floor(ext / cell) + 1;
-                if (c < 1) c = 1;
-                g.dims[d] = (int)std::min<long long>(c, 1 << 30);
-                g.origin[d] = bb[2 * d];
-            } else {
-                g.dims[d] = 1;
-                g.origin[d] = 0.0;
-            }
-            tot *= g.dims[d];
+        double tot = 1.0, c[3] = {1.0, 1.0, 1.0};
+        for (int d = 0; d < dim && d < 3; d++) {
+            const double ext = bb[2 * d + 1] - bb[2 * d];
+            c[d] = (std::isfinite(ext) && ext > 0.0) ? std::floor(ext / cell) + 1.0 : 1.0;
+            tot *= c[d];
         }
-        if (tot <= std::max<long long>(4LL * n, 1 << 20)) { g.ncell = tot; break; }
+        if (tot <= limit || !std::isfinite(cell)) {
+            long long t = 1;
+            for (int d = 0; d < 3; d++) {
+                g.dims[d] = d < dim ? (int)std::min(c[d], limit) : 1;
+                g.origin[d] = d < dim ? bb[2 * d] : 0.0;
+                t *= g.dims[d];
+            }
+            g.ncell = t;
+            break;
+        }
         cell *= 1.5;
     }
     g.inv_cell = 1.0 / cell;
@@ -203,7 +209,8 @@ __device__ __forceinline__ int cell_coord(double x, double o, double inv, int di
     return (int)c;
 }
 
-__global__ void k_cell_count(const double *pts, int n, int dim, Grid g, int *cell_of, int *count) {
+// crowded: raised to a cell's count when it passes FR_MAX_CELL
+__global__ void k_cell_count(const double *pts, int n, int dim, Grid g, int *cell_of, int *count, int *crowded) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c[3] = {0, 0, 0};
@@ -213,7 +220,8 @@ __global__ void k_cell_count(const double *pts, int n, int dim, Grid g, int *cel
     }
     int cell = (c[2] * g.dims[1] + c[1]) * g.dims[0] + c[0];
     cell_of[i] = cell;
-    atomicAdd(&count[cell], 1);
+    const int k = atomicAdd(&count[cell], 1) + 1;
+    if (crowded && k > FR_MAX_CELL) atomicMax(crowded, k);
 }
 
 __global__ void k_cell_scatter(const double *pts, int n, int dim, const int *cell_of, const int *start,
@@ -230,13 +238,35 @@ __global__ void k_cell_scatter(const double *pts, int n, int dim, const int *cel
 }
 
 void bin_points(hipStream_t s, const double *pts, int n, int dim, const Grid &g, int *cell_of, int *cell_count,
-                int *cell_start, int *cell_fill, int *sorted, double *spts, void *scan_tmp) {
+                int *cell_start, int *cell_fill, int *sorted, double *spts, void *scan_tmp, int *crowded) {
     HIP_OK(hipMemsetAsync(cell_count, 0, sizeof(int) * g.ncell, s));
     HIP_OK(hipMemsetAsync(cell_fill, 0, sizeof(int) * g.ncell, s));
-    hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, s, pts, n, dim, g, cell_of, cell_count);
+    hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, s, pts, n, dim, g, cell_of, cell_count,
+                       crowded);
     exclusive_scan_i32(s, cell_count, cell_start, g.ncell, scan_tmp);
     hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, pts, n, dim, cell_of, cell_start,
                        cell_fill, sorted, spts);
+}
+
+// One thread per cell: insertion sort of the cell's indices (cells hold a
+// few tens of points; this runs only when particles are renumbered).  A cell
+// past FR_MAX_CELL is left as it is: the search of that step reports it.
+__global__ void k_cell_order(const int *start, long long ncell, int *sorted) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int b = start[c], e = start[c + 1];
+    if (e - b > FR_MAX_CELL) return;
+    for (int i = b + 1; i < e; i++) {
+        const int v = sorted[i];
+        int j = i - 1;
+        while (j >= b && sorted[j] > v) { sorted[j + 1] = sorted[j]; j--; }
+        sorted[j + 1] = v;
+    }
+}
+
+void cell_order_by_index(hipStream_t s, const int *cell_start, long long ncell, int *sorted) {
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL(k_cell_order, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, cell_start, ncell, sorted);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,6 +700,14 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
     const bool live = t < m;
     int qi = live ? (order ? order[t] : t) : 0;
     int cnt = 0;
+    if (err->crowded > 0) {
+        // the binning put more than FR_MAX_CELL points in one cell (a point
+        // far from the rest coarsened the grid): no lists, and the step
+        // reports it instead of scanning those cells for every query
+        if (live) deg[qi] = 0;
+        if (t == 0) atomicCAS(&err->code, 0, (int)SSASDPD_ERR_CAPACITY);
+        return;
+    }
     if (live) {
         auto LI = [&](int k) -> int & { return l_idx[k * WAVE + tl]; };
         auto LD = [&](int k) -> double & { return l_r2[k * WAVE + tl]; };

@@ -415,4 +415,28 @@ void sdpd_correct(hipStream_t s, const SdpdArgs &a) {
 void sdpd_bvf(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_bvf, grid1(a.n), dim3(256), 0, s, a); }
 void sdpd_pack(hipStream_t s, const SdpdArgs &a) { hipLaunchKernelGGL(k_pack, grid1(a.n), dim3(256), 0, s, a); }
 
+
+// Row permutation: gather dst[p][c] = src[map[p]][c], or scatter
+// dst[map[p]][c] = src[p][c]; rows of `cols` elements of T.
+template <class T>
+__global__ void k_permute_rows(int n, int cols, const int *__restrict__ map, const T *__restrict__ src,
+                               T *__restrict__ dst, int scatter) {
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * cols) return;
+    const int p = (int)(e / cols), c = (int)(e % cols);
+    const size_t q = (size_t)map[p] * cols + c;
+    if (scatter) dst[q] = src[e];
+    else dst[e] = src[q];
+}
+void permute_rows(hipStream_t s, int n, int cols, int esz, const int *map, const void *src, void *dst, bool scatter) {
+    if (n <= 0 || cols <= 0) return;
+    const size_t m = (size_t)n * cols;
+    const dim3 g((unsigned)((m + 255) / 256));
+    if (esz == 8)
+        hipLaunchKernelGGL(k_permute_rows<uint64_t>, g, dim3(256), 0, s, n, cols, map, (const uint64_t *)src,
+                           (uint64_t *)dst, scatter ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_permute_rows<uint32_t>, g, dim3(256), 0, s, n, cols, map, (const uint32_t *)src,
+                           (uint32_t *)dst, scatter ? 1 : 0);
+}
 }  // namespace ssa

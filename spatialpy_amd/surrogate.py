@@ -201,15 +201,19 @@ class Surrogate:
 
     # engine ----------------------------------------------------------------
     def attach(self, engine, check_every=50, max_rel_err=0.02, refresh_iters=0, refresh_lr=1e-4, keep_samples=32,
-               max_particles=32768, rearm=False):
+               max_particles=32768, rearm=False, cap_factor=20.0):
         """Use the network on engine's steps between checks every
         check_every steps (fallback above max_rel_err).  refresh_iters > 0:
         each check adds that plain step as a sample (keeping the newest
         keep_samples) and takes refresh_iters Adam steps.  rearm: a failed
-        check pauses the network only until a later check passes."""
+        check pauses the network only until a later check passes.
+        cap_factor: between checks a particle's predicted pair acceleration
+        is capped at this multiple of the last check's RMS plain pair
+        acceleration (<= 0: no cap)."""
         _abi.check(self.lib, self.lib.ssasdpd_set_surrogate(engine.h, self.h, int(check_every), float(max_rel_err)),
                    "set_surrogate")
         _abi.check(self.lib, self.lib.ssasdpd_set_surrogate_rearm(engine.h, 1 if rearm else 0), "set_surrogate_rearm")
+        _abi.check(self.lib, self.lib.ssasdpd_set_surrogate_cap(engine.h, float(cap_factor)), "set_surrogate_cap")
         _abi.check(self.lib, self.lib.ssasdpd_set_surrogate_refresh(engine.h, int(refresh_iters), float(refresh_lr),
                                                                     int(keep_samples), int(max_particles)),
                    "set_surrogate_refresh")
@@ -226,8 +230,14 @@ class Surrogate:
         act = C.c_int32()
         engine.lib.ssasdpd_surrogate_stats(engine.h, C.byref(a), C.byref(b), C.byref(e), C.byref(m), C.byref(r),
                                            C.byref(f), C.byref(act))
+        nc, cap = C.c_uint64(), C.c_double()
+        _abi.check(engine.lib, engine.lib.ssasdpd_surrogate_capped(engine.h, C.byref(nc), C.byref(cap)),
+                   "surrogate_capped")
+        # capped: network predictions scaled down to the cap (a multiple of
+        # the last check's RMS plain pair acceleration) since the attach
         return {"nn_steps": a.value, "checks": b.value, "last_err": e.value, "max_err": m.value,
-                "refreshes": r.value, "failed_checks": f.value, "active": bool(act.value)}
+                "refreshes": r.value, "failed_checks": f.value, "active": bool(act.value),
+                "capped": nc.value, "cap": cap.value}
 
 
 def fit_scales(engine, sa):

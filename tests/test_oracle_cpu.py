@@ -239,6 +239,38 @@ def test_library_exports_every_declared_function():
         assert hasattr(lib, nm), nm
 
 
+def _grid_shape(lib, bbox, n, dim, h):
+    bb = (ctypes.c_double * 6)(*bbox)
+    dims = (ctypes.c_int32 * 3)()
+    nc, cell = ctypes.c_int64(), ctypes.c_double()
+    assert lib.ssasdpd_grid_shape(bb, n, dim, h, dims, ctypes.byref(nc), ctypes.byref(cell)) == 0
+    return list(dims), nc.value, cell.value
+
+
+def test_search_grid_stays_bounded_for_far_points():
+    """The search grid for a bounding box: cells of side 1.01 h over an
+    ordinary box, and a coarser grid of at most max(4n, 2^20) cells, never
+    a wrapped count, when one point lies far from the rest (a box of 1e6 or
+    1e8 units at h = 0.03 needs 1e22-1e27 cells of side h; the cell count
+    wrapped to a negative number or to 0 in 64-bit integers)."""
+    from spatialpy_amd import _abi
+    lib = _abi.load_library()
+    n, h = 1_005_649, 0.030223631139250296
+    dims, nc, cell = _grid_shape(lib, [-1.0, 1.0, -0.5, 0.5, -0.25, 0.25], n, 3, h)
+    assert cell == pytest.approx(1.01 * h, rel=1e-12)
+    assert dims == [int(2.0 // cell) + 1, int(1.0 // cell) + 1, int(0.5 // cell) + 1]
+    assert nc == dims[0] * dims[1] * dims[2]
+    for far in (1e3, 1e6, 1e8, 1e12, 1e300):
+        dims, nc, cell = _grid_shape(lib, [-far, 1.0, -far, 1.0, -far, 1.0], n, 3, h)
+        assert 1 <= nc <= max(4 * n, 1 << 20), (far, dims, nc)
+        assert nc == dims[0] * dims[1] * dims[2] and min(dims) >= 1, (far, dims, nc)
+        # the cells cover the box
+        assert all(d * cell >= far for d in dims), (far, dims, cell)
+    # non-finite extents (NaN / Inf positions) get one cell on that side
+    dims, nc, _ = _grid_shape(lib, [0.0, float("nan"), 0.0, float("inf"), 0.0, 1.0], 100, 3, 0.1)
+    assert dims[:2] == [1, 1] and nc == dims[2]
+
+
 def test_engine_fails_loudly_without_gpu():
     from conftest import gpu_available
     if gpu_available():
