@@ -107,6 +107,13 @@ struct ssasdpd_engine {
     hipStream_t ps = nullptr;
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
+    // A renumber inside the look-ahead: the event loop beside it reads the
+    // types from rtype (a copy made before it, ev_rt) while the particle
+    // arrays are permuted; the counts are permuted after it on the engine
+    // stream, by the order kept in rmap (ev_map), through xbuf.
+    int *rtype = nullptr, *rmap = nullptr;
+    uint32_t *xbuf = nullptr;
+    hipEvent_t ev_rt = nullptr, ev_map = nullptr;
     // Wall-clock limit of one run / run_batch / advance call (0: none),
     // checked between steps and between the segments of a fused static run.
     double timeout_s = 0;
@@ -262,6 +269,10 @@ struct ssasdpd_engine {
         dfree(deg_alt); dfree(erow_alt);
         if (la_done) (void)hipEventDestroy(la_done);
         la_done = nullptr;
+        if (ev_rt) (void)hipEventDestroy(ev_rt);
+        if (ev_map) (void)hipEventDestroy(ev_map);
+        ev_rt = ev_map = nullptr;
+        dfree(rtype); dfree(rmap); dfree(xbuf);
         if (ps) (void)hipStreamDestroy(ps);
         ps = nullptr;
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp); dfree(rec);
@@ -492,6 +503,21 @@ struct ssasdpd_engine {
         perm_h.resize(n);
         for (int i = 0; i < n; i++) perm_h[i] = i;
         h2d(st, perm, perm_h.data(), n);
+        {
+            // the renumber's staging buffer at its largest row size, so no
+            // renumber (one beside a running event loop included) reallocates
+            const size_t row = std::max<size_t>({8, (size_t)8 * std::max(d->n_chem_species, 0),
+                                                 (size_t)8 * std::max(d->n_data_fn, 0),
+                                                 (size_t)4 * std::max(d->n_species, 0)});
+            grow(pbuf, pbuf_cap, (size_t)n * row);
+        }
+        if (ps && d->n_species > 0 && !d->static_domain) {
+            rtype = dalloc<int>(n);
+            rmap = dalloc<int>(n);
+            xbuf = dalloc<uint32_t>((size_t)n * d->n_species);
+            HIP_OK(hipEventCreateWithFlags(&ev_rt, hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&ev_map, hipEventDisableTiming));
+        }
         any_solid = std::any_of(so.begin(), so.end(), [](int v) { return v != 0; });
         movers.clear();
         for (int i = 0; i < n; i++)
@@ -694,16 +720,28 @@ struct ssasdpd_engine {
         permute_rows(st, n, cols, esz, map, arr, tmp, false);
         HIP_OK(hipMemcpyAsync(arr, tmp, bytes, hipMemcpyDeviceToDevice, st));
     }
+    // The new order: cell order of the current positions, ties inside a
+    // cell by index, so the numbering (which the random decisions of the
+    // chemistry see) is a function of the positions.  m[p]: the particle
+    // that becomes p; valid until the next binning.
+    const int *renumber_map() {
+        bin_tree_points();
+        cell_order_by_index(st, cell_start, grid.ncell, sorted);
+        return sorted;
+    }
     // Put the particles in cell order of their current positions: every
     // array that carries state from one step to the next, the counts of
     // n_traj batch trajectories, and the map to the caller's numbering.
     // Lists, rates and the queue are rebuilt by the next step.
     void renumber(int n_traj = 0) {
-        bin_tree_points();                        // sorted[p]: the p-th particle in cell order
-        // ties inside a cell by index, so the numbering (which the random
-        // decisions of the chemistry see) is a function of the positions
-        cell_order_by_index(st, cell_start, grid.ncell, sorted);
-        const int *m = sorted;
+        const int *m = renumber_map();
+        permute_particles(m);
+        if (S > 0) permute_inplace(xx, S, 4, m);
+        for (int b = 0; b < n_traj; b++) permute_inplace(b_xx + (size_t)b * n * S, S, 4, m);
+    }
+    // every per-particle array but the counts, and the map to the caller's
+    // numbering
+    void permute_particles(const int *m) {
         for (double *a : {x, v, vt, F, Fbp, normal})
             for (int k = 0; k < 3; k++) permute_inplace(a + (size_t)k * n, 1, 8, m);
         for (double *a : {rho, old_rho, Frho, mass, nu, bvf}) permute_inplace(a, 1, 8, m);
@@ -713,8 +751,6 @@ struct ssasdpd_engine {
             permute_inplace(C, Sc, 8, m);
             permute_inplace(Q, Sc, 8, m);
         }
-        if (S > 0) permute_inplace(xx, S, 4, m);
-        for (int b = 0; b < n_traj; b++) permute_inplace(b_xx + (size_t)b * n * S, S, 4, m);
         if (n_df > 0) permute_inplace(dfn, n_df, 8, m);
         permute_rows(st, n, 1, 4, m, perm, perm_tmp, false);
         std::swap(perm, perm_tmp);
@@ -1041,7 +1077,7 @@ struct ssasdpd_engine {
     // its particle phase may start now, beside this step's event loop.
     int do_step(bool lookahead = false) {
         const unsigned step = cur;
-        // (a renumber step is never prepared by a look-ahead, see below)
+        // (a renumber step prepared by a look-ahead was renumbered there)
         if (!la_pending && renumber_due(step)) renumber();
         if (la_pending) {
             HIP_OK(hipStreamWaitEvent(st, la_done, 0));
@@ -1060,24 +1096,47 @@ struct ssasdpd_engine {
                 QueueState q;
                 d2h(st, &q, qs, 1);
                 // this step's particle errors, read before a look-ahead can add the next step's
-                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt && !renumber_due(step + 1);
+                // a renumber step is prepared too, unless the event loop reads
+                // data-function values (those are permuted with the particles)
+                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt &&
+                                  (!renumber_due(step + 1) || (rtype && n_df == 0));
                 DevErr pe{};
                 if (want) d2h(st, &pe, derr, 1);
                 HIP_OK(hipStreamSynchronize(st));
                 if (q.initialized == 1) rdme_initialized = true;
                 la = want && pe.code == 0;
             }
+            if (la && renumber_due(step + 1)) {
+                // the look-ahead permutes type beside this event loop
+                HIP_OK(hipMemcpyAsync(rtype, type, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+                HIP_OK(hipEventRecord(ev_rt, st));
+                ra.type = rtype;
+            }
             ph_begin(4);
             rdme_take_step(st, ra);
             ph_end(4);
         }
         if (la) {
+            const bool ren = renumber_due(step + 1);
             std::swap(deg, deg_alt);
             std::swap(erow, erow_alt);
             std::swap(st, ps);
+            if (ren) {
+                HIP_OK(hipStreamWaitEvent(st, ev_rt, 0));
+                const int *m = renumber_map();
+                HIP_OK(hipMemcpyAsync(rmap, m, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+                HIP_OK(hipEventRecord(ev_map, st));
+                permute_particles(m);
+            }
             particle_phase(step + 1);
             HIP_OK(hipEventRecord(la_done, st));
             std::swap(st, ps);
+            if (ren) {
+                // the counts, once this step's event loop is done with them
+                HIP_OK(hipStreamWaitEvent(st, ev_map, 0));
+                permute_rows(st, n, S, 4, rmap, xx, xbuf, false);
+                HIP_OK(hipMemcpyAsync(xx, xbuf, sizeof(uint32_t) * n * S, hipMemcpyDeviceToDevice, st));
+            }
             la_pending = true;
         }
         int rc = check(step, !la);

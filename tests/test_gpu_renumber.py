@@ -73,6 +73,37 @@ def test_renumbered_run_follows_the_plain_run(make):
     assert np.array_equal(r2["x"], r1["x"])
 
 
+@pytest.mark.parametrize("case", ["dam_break", "cylinder20k"])
+def test_renumber_beside_the_event_loop_equals_the_serial_one(case):
+    """advance() renumbers inside its look-ahead (the next step's particle
+    phase on a second stream, beside this step's event loop: the particles
+    are permuted there, the counts after the event loop); run() renumbers
+    between steps.  Same seed, same state bit for bit.  advance(20) ends a
+    call on the step before a renumber (no look-ahead there), so step 20 is
+    renumbered between steps and step 40 beside the event loop."""
+    if case == "dam_break":
+        sa = workloads.dam_break_2d(nx=24, ny=24, spacing=0.025, n_mol=200, nt=45, out_every=45, dt=2e-4)
+    else:
+        sa = workloads.cylinder_3d(n_target=20_000, nt=45, out_every=45)
+    print(f"renumber beside the event loop: {case} n={sa.n}", flush=True)
+    fields = ("xx", "x", "v", "rho")
+    ref = Engine(sa).run(seed=11, fields=fields)
+    eng = Engine(sa)
+    eng.reset(seed=11)
+    eng.advance(20)
+    eng.advance(25)
+    st = eng.state()
+    assert np.array_equal(st["xx"], ref["xx"][-1])
+    for f in ("x", "v", "rho"):
+        assert np.array_equal(st[f].view(np.uint64), ref[f][-1].view(np.uint64)), f
+    # and in one call (renumbers at 20 and 40 beside the event loop)
+    eng.reset(seed=11)
+    eng.advance(45)
+    st2 = eng.state()
+    assert np.array_equal(st2["xx"], ref["xx"][-1])
+    assert np.array_equal(st2["x"].view(np.uint64), ref["x"][-1].view(np.uint64))
+
+
 def test_renumbered_chemistry_agrees_in_distribution():
     """The moving dam break's species (diffusion and its reaction): 64
     renumbered and 64 plain trajectories (one batch each), totals and the
