@@ -114,6 +114,10 @@ struct ssasdpd_engine {
     int *rtype = nullptr, *rmap = nullptr;
     uint32_t *xbuf = nullptr;
     hipEvent_t ev_rt = nullptr, ev_map = nullptr;
+    // The particle phase also forms vol and Ddiag (the particle part of the
+    // rates); the event loop reads them, so a look-ahead writes the other
+    // pair, swapped in with deg and erow.
+    double *Ddiag_alt = nullptr, *vol_alt = nullptr;
     // Wall-clock limit of one run / run_batch / advance call (0: none),
     // checked between steps and between the segments of a fused static run.
     double timeout_s = 0;
@@ -272,7 +276,7 @@ struct ssasdpd_engine {
         if (ev_rt) (void)hipEventDestroy(ev_rt);
         if (ev_map) (void)hipEventDestroy(ev_map);
         ev_rt = ev_map = nullptr;
-        dfree(rtype); dfree(rmap); dfree(xbuf);
+        dfree(rtype); dfree(rmap); dfree(xbuf); dfree(Ddiag_alt); dfree(vol_alt);
         if (ps) (void)hipStreamDestroy(ps);
         ps = nullptr;
         dfree(xtree); dfree(spts); dfree(d_bbox); dfree(d_bbtmp); dfree(rec);
@@ -515,6 +519,8 @@ struct ssasdpd_engine {
             rtype = dalloc<int>(n);
             rmap = dalloc<int>(n);
             xbuf = dalloc<uint32_t>((size_t)n * d->n_species);
+            Ddiag_alt = dalloc<double>((size_t)n * d->n_species);
+            vol_alt = dalloc<double>(n);
             HIP_OK(hipEventCreateWithFlags(&ev_rt, hipEventDisableTiming));
             HIP_OK(hipEventCreateWithFlags(&ev_map, hipEventDisableTiming));
         }
@@ -720,12 +726,13 @@ struct ssasdpd_engine {
         permute_rows(st, n, cols, esz, map, arr, tmp, false);
         HIP_OK(hipMemcpyAsync(arr, tmp, bytes, hipMemcpyDeviceToDevice, st));
     }
-    // The new order: cell order of the current positions, ties inside a
-    // cell by index, so the numbering (which the random decisions of the
-    // chemistry see) is a function of the positions.  m[p]: the particle
+    // The new order: cell order of the current positions, cells taken tile
+    // by tile (4 x 4 x 4 in 3D), ties inside a cell by index, so the
+    // numbering (which the random decisions of the chemistry see) is a
+    // function of the positions.  m[p]: the particle
     // that becomes p; valid until the next binning.
     const int *renumber_map() {
-        bin_tree_points();
+        bin_tree_points(true);
         cell_order_by_index(st, cell_start, grid.ncell, sorted);
         return sorted;
     }
@@ -776,13 +783,21 @@ struct ssasdpd_engine {
     }
     int orig_index(int p) { return (renumbered && p >= 0 && p < n) ? perm_host()[p] : p; }
 
-    void bin_tree_points() {
+    // tiled: cells numbered tile by tile (a binning for the renumbering
+    // only; the search needs the plain numbering)
+    void bin_tree_points(bool tiled = false) {
         ph_begin(0);
         HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
         compute_bbox(st, xtree, n, dim, d_bbox, d_bbtmp);
         d2h(st, h_bbox, d_bbox, 6);
         HIP_OK(hipStreamSynchronize(st));
         grid_setup(h_bbox, n, dim, h, grid);
+        if (tiled) {
+            Grid tg = grid;
+            grid_tile(tg, dim);
+            // within what the scan's work space was sized for
+            if (tg.ncell <= 4LL * n + (1 << 20)) grid = tg;
+        }
         if (grid.ncell + 1 > cell_cap) {
             // hipFree waits for the whole device (a running event loop too), so
             // grow with a quarter of headroom: a slowly widening bounding box
@@ -984,6 +999,12 @@ struct ssasdpd_engine {
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);
         ph_end(2);
+        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME) && (step == 0 || moving)) {
+            // the particle part of this step's rates (vol, Ddiag)
+            ph_begin(3);
+            rdme_diag(st, rdme_args(step));
+            ph_end(3);
+        }
     }
 
     void end_step() {
@@ -1098,7 +1119,7 @@ struct ssasdpd_engine {
                 // this step's particle errors, read before a look-ahead can add the next step's
                 // a renumber step is prepared too, unless the event loop reads
                 // data-function values (those are permuted with the particles)
-                const bool want = lookahead && ps && static_domain == 0 && !nn && step + 1 < nt &&
+                const bool want = lookahead && ps && Ddiag_alt && static_domain == 0 && !nn && step + 1 < nt &&
                                   (!renumber_due(step + 1) || (rtype && n_df == 0));
                 DevErr pe{};
                 if (want) d2h(st, &pe, derr, 1);
@@ -1120,6 +1141,8 @@ struct ssasdpd_engine {
             const bool ren = renumber_due(step + 1);
             std::swap(deg, deg_alt);
             std::swap(erow, erow_alt);
+            std::swap(Ddiag, Ddiag_alt);
+            std::swap(vol, vol_alt);
             std::swap(st, ps);
             if (ren) {
                 HIP_OK(hipStreamWaitEvent(st, ev_rt, 0));

@@ -104,7 +104,16 @@ struct Grid {
     double inv_cell;
     int dims[3];
     long long ncell;
+    // tile[0] == 0: cell (cx, cy, cz) is (cz * dims[1] + cy) * dims[0] + cx,
+    // as the search reads it.  Otherwise (a binning for the renumbering
+    // only) cells are numbered tile by tile, tiles of tile[d] cells per
+    // side, x fastest within a tile and among tiles; ncell counts the key
+    // space, partial tiles included.
+    int tile[3];
+    int ntile[3];
 };
+// The tiled numbering of g's cells (see Grid).
+void grid_tile(Grid &g, int dim);
 
 // What the neighbour loops gather from particle j, packed after the
 // predictor: one 128-byte line per neighbour instead of a load from each of
@@ -277,6 +286,9 @@ struct RdmeArgs {
     const double *data_fn;
     int n_df;
 };
+// vol and Ddiag from the particle state and this step's lists (once per
+// step, before rdme_rates, which reads them)
+void rdme_diag(hipStream_t s, const RdmeArgs &a);
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);
 void rdme_take_step(hipStream_t s, const RdmeArgs &a, int n_traj = 1);

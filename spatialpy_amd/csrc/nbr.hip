@@ -192,6 +192,7 @@ void grid_setup(const double *bb, int n, int dim, double h, Grid &g) {
             for (int d = 0; d < 3; d++) {
                 g.dims[d] = d < dim ? (int)std::min(c[d], limit) : 1;
                 g.origin[d] = d < dim ? bb[2 * d] : 0.0;
+                g.tile[d] = g.ntile[d] = 0;
                 t *= g.dims[d];
             }
             g.ncell = t;
@@ -209,6 +210,26 @@ __device__ __forceinline__ int cell_coord(double x, double o, double inv, int di
     return (int)c;
 }
 
+// Tiles of 4 x 4 x 4 cells in 3D, 8 x 8 in 2D, 64 in 1D (a few hundred
+// points each at the search's cell size), narrower where the grid is.
+void grid_tile(Grid &g, int dim) {
+    const int T = dim >= 3 ? 4 : (dim == 2 ? 8 : 64);
+    long long k = 1;
+    for (int d = 0; d < 3; d++) {
+        g.tile[d] = d < dim ? std::max(1, std::min(T, g.dims[d])) : 1;
+        g.ntile[d] = (g.dims[d] + g.tile[d] - 1) / g.tile[d];
+        k *= (long long)g.ntile[d] * g.tile[d];
+    }
+    g.ncell = k;
+}
+
+__device__ __forceinline__ int cell_key(const int c[3], const Grid &g) {
+    if (g.tile[0] == 0) return (c[2] * g.dims[1] + c[1]) * g.dims[0] + c[0];
+    const int t = (c[2] / g.tile[2] * g.ntile[1] + c[1] / g.tile[1]) * g.ntile[0] + c[0] / g.tile[0];
+    const int w = ((c[2] % g.tile[2]) * g.tile[1] + c[1] % g.tile[1]) * g.tile[0] + c[0] % g.tile[0];
+    return t * (g.tile[0] * g.tile[1] * g.tile[2]) + w;
+}
+
 // crowded: raised to a cell's count when it passes FR_MAX_CELL
 __global__ void k_cell_count(const double *pts, int n, int dim, Grid g, int *cell_of, int *count, int *crowded) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -218,7 +239,7 @@ __global__ void k_cell_count(const double *pts, int n, int dim, Grid g, int *cel
         int v = cell_coord(pts[(size_t)d * n + i], g.origin[d], g.inv_cell, g.dims[d]);
         c[d] = min(max(v, 0), g.dims[d] - 1);
     }
-    int cell = (c[2] * g.dims[1] + c[1]) * g.dims[0] + c[0];
+    const int cell = cell_key(c, g);
     cell_of[i] = cell;
     const int k = atomicAdd(&count[cell], 1) + 1;
     if (crowded && k > FR_MAX_CELL) atomicMax(crowded, k);
@@ -716,12 +737,12 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
         const int L = W < SLOTS ? W : SLOTS;   // LDS slots in use
         double qp[3] = {0.0, 0.0, 0.0};
         for (int d = 0; d < dim; d++) qp[d] = q[(size_t)d * m + qi];
-        int c[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         for (int d = 0; d < 3; d++) {
             if (d < dim) {
-                c[d] = cell_coord(qp[d], g.origin[d], g.inv_cell, g.dims[d]);
-                lo[d] = max(c[d] - 1, 0);
-                hi[d] = min(c[d] + 1, g.dims[d] - 1);
+                const int c = cell_coord(qp[d], g.origin[d], g.inv_cell, g.dims[d]);
+                lo[d] = max(c - 1, 0);
+                hi[d] = min(c + 1, g.dims[d] - 1);
             }
         }
         const double sq = h * h;
@@ -789,35 +810,48 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                 sort_list(GI, GD, cnt, kd, qp);
             }
             if (has_fin) {
-                // r, D_ij, the neighbour's type and the voxel-major row, in list order
+                // r, D_ij, the neighbour's type and the voxel-major row, in
+                // list order; the neighbours' records are fetched FB at a
+                // time ahead of the stores (the compiler cannot move a load
+                // above a store through these pointers)
                 const double mi = fin.mass[qi], ri = fin.rho[qi];
-                for (int k = 0; k < cnt; k++) {
-                    const int j = cnt <= L ? LI(k) : GI(k);
-                    const double rr2 = cnt <= L ? LD(k) : GD(k);
-                    const double rr = sqrt(rr2);
-                    double mj, rj;
-                    int tj;
-                    if (fin.rec) {
-                        const PRec *pj = fin.rec + j;
-                        const double2 rm = *(const double2 *)&pj->rho;
-                        rj = rm.x;
-                        mj = rm.y;
-                        tj = pj->type;
-                    } else {
-                        mj = fin.mass[j];
-                        rj = fin.rho[j];
-                        tj = fin.type[j];
+                constexpr int FB = 4;
+                for (int k0 = 0; k0 < cnt; k0 += FB) {
+                    int jj[FB], tj[FB];
+                    double rr2[FB], rj[FB], mj[FB];
+#pragma unroll
+                    for (int u = 0; u < FB; u++) {
+                        const int k = k0 + u < cnt ? k0 + u : k0;
+                        jj[u] = cnt <= L ? LI(k) : GI(k);
+                        rr2[u] = cnt <= L ? LD(k) : GD(k);
+                        if (fin.rec) {
+                            const PRec *pj = fin.rec + jj[u];
+                            const double2 rm = *(const double2 *)&pj->rho;
+                            rj[u] = rm.x;
+                            mj[u] = rm.y;
+                            tj[u] = pj->type;
+                        } else {
+                            mj[u] = fin.mass[jj[u]];
+                            rj[u] = fin.rho[jj[u]];
+                            tj[u] = fin.type[jj[u]];
+                        }
                     }
-                    const double D = edge_dij(mi, ri, mj, rj, rr2, rr, h);
-                    if (D != D) {
-                        atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
-                        atomicMin(&err->particle, qi);
+#pragma unroll
+                    for (int u = 0; u < FB; u++) {
+                        const int k = k0 + u;
+                        if (k >= cnt) break;
+                        const double rr = sqrt(rr2[u]);
+                        const double D = edge_dij(mi, ri, mj[u], rj[u], rr2[u], rr, h);
+                        if (D != D) {
+                            atomicCAS(&err->code, 0, (int)SSASDPD_ERR_NAN_DIJ);
+                            atomicMin(&err->particle, qi);
+                        }
+                        const size_t e = (size_t)k * m + qi;
+                        fin.r[e] = rr;
+                        fin.Dij[e] = D;
+                        fin.ntype[e] = tj[u];
+                        fin.erow[(size_t)qi * W + k] = EdgeRec{jj[u], tj[u], D};
                     }
-                    const size_t e = (size_t)k * m + qi;
-                    fin.r[e] = rr;
-                    fin.Dij[e] = D;
-                    fin.ntype[e] = tj;
-                    fin.erow[(size_t)qi * W + k] = EdgeRec{j, tj, D};
                 }
             }
         }

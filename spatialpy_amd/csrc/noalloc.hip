@@ -114,22 +114,28 @@ __global__ void k_na_csr(int m, int W, const int *deg, const int *row32, const i
 // lies in neighbouring cells) and at most NA_MAX_CELLS cells.  The lists do
 // not depend on the grid: acceptance is exact and the order is (r^2, ANN
 // visit order).
+// Counts in double, as grid_setup: points far apart coarsen the grid
+// instead of wrapping an integer product past the scratch array's cells.
 void grid_capped(const double *bb, int dim, double h, Grid &g) {
     double cell = h * 1.01;
     for (;;) {
-        long long tot = 1;
-        for (int d = 0; d < 3; d++) {
-            if (d < dim) {
-                long long c = (long long)std::floor((bb[2 * d + 1] - bb[2 * d]) / cell) + 1;
-                g.dims[d] = (int)std::max<long long>(1, std::min<long long>(c, 1 << 30));
-                g.origin[d] = bb[2 * d];
-            } else {
-                g.dims[d] = 1;
-                g.origin[d] = 0.0;
-            }
-            tot *= g.dims[d];
+        double tot = 1.0, c[3] = {1.0, 1.0, 1.0};
+        for (int d = 0; d < dim && d < 3; d++) {
+            const double ext = bb[2 * d + 1] - bb[2 * d];
+            c[d] = (std::isfinite(ext) && ext > 0.0) ? std::floor(ext / cell) + 1.0 : 1.0;
+            tot *= c[d];
         }
-        if (tot <= NA_MAX_CELLS) { g.ncell = tot; break; }
+        if (tot <= (double)NA_MAX_CELLS || !std::isfinite(cell)) {
+            long long t = 1;
+            for (int d = 0; d < 3; d++) {
+                g.dims[d] = d < dim ? (int)std::min(c[d], (double)NA_MAX_CELLS) : 1;
+                g.origin[d] = d < dim ? bb[2 * d] : 0.0;
+                g.tile[d] = g.ntile[d] = 0;
+                t *= g.dims[d];
+            }
+            g.ncell = t;
+            break;
+        }
         cell *= 1.5;
     }
     g.inv_cell = 1.0 / cell;

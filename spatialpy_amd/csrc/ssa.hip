@@ -200,20 +200,33 @@ __global__ void k_rates(RdmeArgs a) {
         sr += p;
     }
     srrate[i] = sr;
-    if (b == 0) a.vol[i] = a.mass[i] / a.rho[i];
     double sd = 0.0;
+    for (int s = 0; s < S; s++) sd += a.Ddiag[(size_t)i * S + s] * xv[s];
+    sdrate[i] = sd;
+}
+// The particle part of the rates, shared by every trajectory: vol and the
+// per-species diffusion sums over the neighbour row (Ddiag), from this
+// step's lists; the particle phase runs it (beside the previous step's
+// event loop when a look-ahead runs it).
+__global__ void k_diag(RdmeArgs a) {
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    const int i = chunk * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    a.vol[i] = a.mass[i] / a.rho[i];
     const int dg = a.deg[i];
-    for (int s = 0; s < S; s++) {
+    for (int s = 0; s < a.S; s++) {
         double dd = 0.0;
         for (int k = 0; k < dg; k++) {
-            size_t e = (size_t)k * a.n + i;
-            double dc = a.Dmat[(size_t)s * a.T + (a.ntype[e] - 1)];   // ntype[e] = type of neighbour e
+            const size_t e = (size_t)k * a.n + i;
+            const double dc = a.Dmat[(size_t)s * a.T + (a.ntype[e] - 1)];   // ntype[e] = type of neighbour e
             dd += dc * a.Dij[e];
         }
-        if (b == 0) a.Ddiag[(size_t)i * S + s] = dd;
-        sd += dd * xv[s];
+        a.Ddiag[(size_t)i * a.S + s] = dd;
     }
-    sdrate[i] = sd;
+}
+void rdme_diag(hipStream_t s, const RdmeArgs &a) {
+    hipLaunchKernelGGL(k_diag, xcd_grid((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj) {
     const dim3 g(xcd_grid((a.n + 255) / 256).x, n_traj);
