@@ -56,7 +56,10 @@ __device__ __forceinline__ void store_rec_to(const SdpdArgs &a, int i, double2 *
     p[4] = make_double2(a.vt[2 * n + i], a.nu[i]);
     p[5] = make_double2(a.rho[i], a.mass[i]);
     ((int4 *)p)[6] = make_int4(a.type[i], a.solid[i], 0, 0);
-    p[7] = make_double2(0.0, 0.0);
+    // what every pair with this particle as j forms from j alone, by the
+    // same operations as the pair loop: P_j / rho_j^2 and m_j / rho_j
+    const double rho = a.rho[i], P = a.P0 * (rho / a.rho0 - 1.0);
+    p[7] = make_double2(P / (rho * rho), a.mass[i] / rho);
 }
 __device__ __forceinline__ void store_rec(const SdpdArgs &a, int i) { store_rec_to(a, i, (double2 *)(a.rec + i)); }
 
@@ -73,6 +76,8 @@ __device__ __forceinline__ void load_rec(const PRec *r, double x[3], double v[3]
 }
 
 __device__ __forceinline__ double2 load_rho_mass(const PRec *r) { return *(const double2 *)&r->rho; }
+// P_j / rho_j^2 and m_j / rho_j of the record (see store_rec_to)
+__device__ __forceinline__ double2 load_rec_pm(const PRec *r) { return ((const double2 *)r)[7]; }
 
 __global__ void k_pack(SdpdArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
@@ -195,8 +200,10 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
         const Real R = r / hR;
         const Real dWdr = alpha * ((Real)-12 * r / (hR * hR)) * (((Real)1 - R) * ((Real)1 - R));
         double xj[3], vj[3], vtj[3], nujd, rhojd, mjd;
+        double2 pmj = make_double2(0.0, 0.0);
         if (momentum) {
             load_rec(a.rec + j, xj, vj, vtj, nujd, rhojd, mjd);
+            pmj = load_rec_pm(a.rec + j);
         } else {
             const double2 rm = load_rho_mass(a.rec + j);
             rhojd = rm.x;
@@ -214,13 +221,15 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
             }
             const Real vjR[3] = {(Real)vj[0], (Real)vj[1], (Real)vj[2]};
             const Real dvtj[3] = {(Real)(vj[0] - vtj[0]), (Real)(vj[1] - vtj[1]), (Real)(vj[2] - vtj[2])};
-            const Real Pj = P0 * (rhoj / rho0 - (Real)1.0);
-            Real pg = Pi / (rhoi * rhoi) + Pj / (rhoj * rhoj);
-            if (pg < 0) pg = -Pi / (rhoi * rhoi) + Pj / (rhoj * rhoj);
+            // P_j / rho_j^2 and m_j / rho_j from the record: the same
+            // operations on the same values as here, done once per particle
+            const Real Pj_r = (Real)pmj.x, mrj = (Real)pmj.y;
+            Real pg = Pi / (rhoi * rhoi) + Pj_r;
+            if (pg < 0) pg = -Pi / (rhoi * rhoi) + Pj_r;
             const Real fp = (Real)-1.0 * mj * pg * dWdr / (r + (Real)0.001 * hR);
             const Real fv = mj * ((Real)2.0 * (nui * nuj) / (nui + nuj)) * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr /
                             ((rhoi * rhoj));
-            const Real mri = mi / rhoi, mrj = mj / rhoj;
+            const Real mri = mi / rhoi;
             const Real vsum = mri * mri + mrj * mrj;
             const Real fbp = (Real)-10.0 * P0 * ((Real)1.0 / mi) * vsum * dWdr / (r + (Real)0.001 * hR);
             Real ft[3];
@@ -236,11 +245,12 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
                 if (!a.skip_F) F[d] += (double)(fp * dx[d] + fv * dv[d] + ft[d]);
                 Fbp[d] += (double)(fbp * dx[d]);
             }
-            Frho = Frho + (double)(rhoi * (mj / rhoj) * dv_dx * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr -
-                                   (Real)0.0 * hR * rhoi * (Real)a.c0 * mj * (Real)2.0 * (rhoj / rhoi - (Real)1.0) *
-                                       (r * r / (r * r + (Real)0.01 * hR * hR)) * ((Real)1.0 / (r + (Real)0.001 * hR)) *
-                                       dWdr / rhoj -
-                                   (mj / rhoj) *
+            // model.cpp:143-146; its middle term (density diffusion) is
+            // multiplied by 0.0 there: for finite states it adds a signed
+            // zero, which leaves the sum (which starts at +0) unchanged, so
+            // it is not formed
+            Frho = Frho + (double)(rhoi * mrj * dv_dx * ((Real)1 / (r + (Real)0.001 * hR)) * dWdr -
+                                   mrj *
                                        (rhoi * (dvti[0] * dx[0] + dvti[1] * dx[1] + dvti[2] * dx[2]) +
                                         rhoj * (dvtj[0] * dx[0] + dvtj[1] * dx[1] + dvtj[2] * dx[2])) *
                                        inv_rh * dWdr);
