@@ -704,6 +704,8 @@ struct ssasdpd_engine {
         movers_stale = false;
     }
 
+    bool rdme_on() const { return S > 0 && !(flags & SSASDPD_FLAG_NO_RDME); }
+
     bool renumber_allowed() const {
         return static_domain == 0 && rng_kind == 0 && trace_cap == 0 &&
                !(flags & (SSASDPD_FLAG_ANN_ORDER_MOVING | SSASDPD_FLAG_NO_RENUMBER));
@@ -830,7 +832,10 @@ struct ssasdpd_engine {
             // coalesced (cell order shares candidate cells but scatters the
             // writes, and measured slower); each sorted list is finished (r,
             // D_ij with the current rho, rows) in the same kernel
-            const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr};
+            // (with the RDME on and few species the finish also forms Ddiag)
+            const bool fin_diag = rdme_on() && S <= FIN_MAX_SPECIES;
+            const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr,
+                                fin_diag ? Ddiag : nullptr, Dmat, S, T};
             // LDS list slots sized by the previous search's longest list
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
                       deg, derr, nullptr, &fa, last_max_deg);
@@ -851,6 +856,7 @@ struct ssasdpd_engine {
         for (int k = 0; k < 3; k++) a.g[k] = g[k];
         a.x = x; a.v = v; a.vt = vt; a.F = F; a.Fbp = Fbp; a.rho = rho; a.old_rho = old_rho; a.Frho = Frho;
         a.bvf = bvf; a.normal = normal; a.mass = mass; a.nu = nu; a.type = type; a.solid = solid; a.C = C; a.Q = Q;
+        a.vol = rdme_on() ? vol : nullptr;
         a.Dmat = Dmat; a.Ndense = Ndense; a.rx = rx; a.bc = bc; a.deg = deg; a.nidx = nidx; a.nr = nr; a.err = derr;
         a.rec = rec;
         a.no_solids = any_solid ? 0 : 1;
@@ -999,8 +1005,8 @@ struct ssasdpd_engine {
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);
         ph_end(2);
-        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME) && (step == 0 || moving)) {
-            // the particle part of this step's rates (vol, Ddiag)
+        if (rdme_on() && S > FIN_MAX_SPECIES && (step == 0 || moving)) {
+            // Ddiag of a model with more species than the search's finish forms
             ph_begin(3);
             rdme_diag(st, rdme_args(step));
             ph_end(3);

@@ -164,6 +164,11 @@ struct EdgeRec {
 // 150-210) plus the neighbour's type and the event loop's voxel-major row,
 // done by the search itself once a query's list is sorted.  Own mass and rho
 // from the arrays; the neighbour's from rec when non-null, else the arrays.
+// Ddiag (optional, models of at most FIN_MAX_SPECIES species): the finish
+// also forms Ddiag[i][s] = sum over the list, in list order, of
+// Dmat[s][type_j - 1] * D_ij (nsm_core__initialize_diff_propensities,
+// simulate_rdme.cpp:131-152), which then needs no pass of its own.
+constexpr int FIN_MAX_SPECIES = 4;
 struct FinishArgs {
     double *r, *Dij;
     int *ntype;
@@ -171,6 +176,9 @@ struct FinishArgs {
     const double *mass, *rho;
     const int *type;
     const PRec *rec;
+    double *Ddiag;
+    const double *Dmat;
+    int S, T;
 };
 void fr_search(hipStream_t s, const double *q, int m, const double *spts, const int *sorted,
                const int *cell_start, int n, int dim, double h, const Grid &g, const KdDev *kd,
@@ -192,6 +200,7 @@ struct SdpdArgs {
     double dt, h, rho0, c0, P0, alpha;
     double g[3];
     double *x, *v, *vt, *F, *Fbp, *rho, *old_rho, *Frho, *bvf, *normal;
+    double *vol;             // mass / rho after the step's last update (RDME), or null
     const double *mass;
     double *nu;
     const int *type, *solid;
@@ -286,8 +295,8 @@ struct RdmeArgs {
     const double *data_fn;
     int n_df;
 };
-// vol and Ddiag from the particle state and this step's lists (once per
-// step, before rdme_rates, which reads them)
+// Ddiag from this step's lists, for models with more than FIN_MAX_SPECIES
+// species (the search's finish forms it otherwise; k_bvf forms vol)
 void rdme_diag(hipStream_t s, const RdmeArgs &a);
 void rdme_rates(hipStream_t s, const RdmeArgs &a, int n_traj = 1);
 void rdme_queue_build(hipStream_t s, const RdmeArgs &a, void *scan_tmp, int *rank_tmp);

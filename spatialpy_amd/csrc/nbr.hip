@@ -815,6 +815,7 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                 // time ahead of the stores (the compiler cannot move a load
                 // above a store through these pointers)
                 const double mi = fin.mass[qi], ri = fin.rho[qi];
+                double dd[FIN_MAX_SPECIES] = {};
                 constexpr int FB = 4;
                 for (int k0 = 0; k0 < cnt; k0 += FB) {
                     int jj[FB], tj[FB];
@@ -851,7 +852,17 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                         fin.Dij[e] = D;
                         fin.ntype[e] = tj[u];
                         fin.erow[(size_t)qi * W + k] = EdgeRec{jj[u], tj[u], D};
+                        if (fin.Ddiag) {
+#pragma unroll
+                            for (int s2 = 0; s2 < FIN_MAX_SPECIES; s2++)
+                                if (s2 < fin.S) dd[s2] += fin.Dmat[(size_t)s2 * fin.T + (tj[u] - 1)] * D;
+                        }
                     }
+                }
+                if (fin.Ddiag) {
+#pragma unroll
+                    for (int s2 = 0; s2 < FIN_MAX_SPECIES; s2++)
+                        if (s2 < fin.S) fin.Ddiag[(size_t)qi * fin.S + s2] = dd[s2];
                 }
             }
         }

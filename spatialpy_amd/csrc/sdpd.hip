@@ -320,11 +320,7 @@ __global__ void k_correct(SdpdArgs a) {
     }
 }
 
-__global__ void k_bvf(SdpdArgs a) {
-    const int chunk = xcd_chunk((a.n + 255) / 256);
-    if (chunk < 0) return;
-    int i = chunk * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+__device__ __forceinline__ void bvf_one(const SdpdArgs &a, int i) {
     const size_t n = a.n;
     if (a.solid[i] == 0 && a.static_domain == 0) {
         const double h = a.h;
@@ -384,6 +380,16 @@ __global__ void k_bvf(SdpdArgs a) {
     }
     for (int s = 0; s < a.Sc; s++) a.C[(size_t)i * a.Sc + s] += a.Q[(size_t)i * a.Sc + s] * a.dt * 0.5;
     apply_bcs(a, i);
+}
+// The BVF, the corrector's species update and the BCs; then vol = mass/rho
+// for the RDME of the step, from the final rho (the BCs may set it)
+__global__ void k_bvf(SdpdArgs a) {
+    const int chunk = xcd_chunk((a.n + 255) / 256);
+    if (chunk < 0) return;
+    const int i = chunk * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    bvf_one(a, i);
+    if (a.vol) a.vol[i] = a.mass[i] / a.rho[i];
 }
 
 static dim3 grid1(int n) { return xcd_grid((n + 255) / 256); }

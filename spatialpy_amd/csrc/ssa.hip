@@ -204,16 +204,15 @@ __global__ void k_rates(RdmeArgs a) {
     for (int s = 0; s < S; s++) sd += a.Ddiag[(size_t)i * S + s] * xv[s];
     sdrate[i] = sd;
 }
-// The particle part of the rates, shared by every trajectory: vol and the
-// per-species diffusion sums over the neighbour row (Ddiag), from this
-// step's lists; the particle phase runs it (beside the previous step's
-// event loop when a look-ahead runs it).
+// The per-species diffusion sums over the neighbour row (Ddiag) of models
+// with more species than the search's finish forms, shared by every
+// trajectory, from this step's lists; the particle phase runs it (beside
+// the previous step's event loop when a look-ahead runs it).
 __global__ void k_diag(RdmeArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
     if (chunk < 0) return;
     const int i = chunk * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    a.vol[i] = a.mass[i] / a.rho[i];
     const int dg = a.deg[i];
     for (int s = 0; s < a.S; s++) {
         double dd = 0.0;
