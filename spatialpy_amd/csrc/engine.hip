@@ -833,7 +833,7 @@ struct ssasdpd_engine {
             // writes, and measured slower); each sorted list is finished (r,
             // D_ij with the current rho, rows) in the same kernel
             // (with the RDME on and few species the finish also forms Ddiag)
-            const bool fin_diag = rdme_on() && S <= FIN_MAX_SPECIES;
+            const bool fin_diag = rdme_on() && S <= FIN_MAX_SPECIES && S * T <= FIN_MAX_DMAT;
             const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr,
                                 fin_diag ? Ddiag : nullptr, Dmat, S, T};
             // LDS list slots sized by the previous search's longest list
@@ -1005,7 +1005,7 @@ struct ssasdpd_engine {
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);
         ph_end(2);
-        if (rdme_on() && S > FIN_MAX_SPECIES && (step == 0 || moving)) {
+        if (rdme_on() && (S > FIN_MAX_SPECIES || S * T > FIN_MAX_DMAT) && (step == 0 || moving)) {
             // Ddiag of a model with more species than the search's finish forms
             ph_begin(3);
             rdme_diag(st, rdme_args(step));

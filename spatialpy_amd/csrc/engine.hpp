@@ -164,11 +164,13 @@ struct EdgeRec {
 // 150-210) plus the neighbour's type and the event loop's voxel-major row,
 // done by the search itself once a query's list is sorted.  Own mass and rho
 // from the arrays; the neighbour's from rec when non-null, else the arrays.
-// Ddiag (optional, models of at most FIN_MAX_SPECIES species): the finish
-// also forms Ddiag[i][s] = sum over the list, in list order, of
-// Dmat[s][type_j - 1] * D_ij (nsm_core__initialize_diff_propensities,
-// simulate_rdme.cpp:131-152), which then needs no pass of its own.
+// Ddiag (optional, models of at most FIN_MAX_SPECIES species and
+// FIN_MAX_DMAT species x type entries): the finish also forms
+// Ddiag[i][s] = sum over the list, in list order, of Dmat[s][type_j - 1] *
+// D_ij (nsm_core__initialize_diff_propensities, simulate_rdme.cpp:131-152),
+// which then needs no pass of its own; Dmat is read from an LDS copy.
 constexpr int FIN_MAX_SPECIES = 4;
+constexpr int FIN_MAX_DMAT = 64;
 struct FinishArgs {
     double *r, *Dij;
     int *ntype;

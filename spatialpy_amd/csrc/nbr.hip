@@ -713,7 +713,12 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
             const int *__restrict__ order, FinishArgs fin, int has_fin) {
     __shared__ int l_idx[SLOTS * WAVE];
     __shared__ double l_r2[SLOTS * WAVE];
+    __shared__ double s_dm[FIN_MAX_DMAT];
     const int tl = (int)threadIdx.x;
+    if (has_fin && fin.Ddiag) {
+        for (int k = tl; k < fin.S * fin.T; k += WAVE) s_dm[k] = fin.Dmat[k];
+        __syncthreads();
+    }
     const int chunk = xcd_chunk((m + WAVE - 1) / WAVE);
     if (chunk < 0) return;
     int t = chunk * WAVE + tl;
@@ -855,7 +860,7 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                         if (fin.Ddiag) {
 #pragma unroll
                             for (int s2 = 0; s2 < FIN_MAX_SPECIES; s2++)
-                                if (s2 < fin.S) dd[s2] += fin.Dmat[(size_t)s2 * fin.T + (tj[u] - 1)] * D;
+                                if (s2 < fin.S) dd[s2] += s_dm[s2 * fin.T + (tj[u] - 1)] * D;
                         }
                     }
                 }
