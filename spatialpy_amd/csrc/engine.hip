@@ -107,6 +107,7 @@ struct ssasdpd_engine {
     hipStream_t ps = nullptr;
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
+    hipEvent_t ev_bbox = nullptr;   // the bounding box has reached the host
     // A renumber inside the look-ahead: the event loop beside it reads the
     // types from rtype (a copy made before it, ev_rt) while the particle
     // arrays are permuted; the counts are permuted after it on the engine
@@ -273,6 +274,8 @@ struct ssasdpd_engine {
         dfree(deg_alt); dfree(erow_alt);
         if (la_done) (void)hipEventDestroy(la_done);
         la_done = nullptr;
+        if (ev_bbox) (void)hipEventDestroy(ev_bbox);
+        ev_bbox = nullptr;
         if (ev_rt) (void)hipEventDestroy(ev_rt);
         if (ev_map) (void)hipEventDestroy(ev_map);
         ev_rt = ev_map = nullptr;
@@ -465,6 +468,7 @@ struct ssasdpd_engine {
             HIP_OK(hipStreamCreate(&ps));
             HIP_OK(hipEventCreateWithFlags(&la_done, hipEventDisableTiming));
         }
+        HIP_OK(hipEventCreateWithFlags(&ev_bbox, hipEventDisableTiming));
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++) {
                 HIP_OK(hipEventCreate(&ev_a[p][k]));
@@ -788,11 +792,22 @@ struct ssasdpd_engine {
     // tiled: cells numbered tile by tile (a binning for the renumbering
     // only; the search needs the plain numbering)
     void bin_tree_points(bool tiled = false) {
-        ph_begin(0);
+        bin_begin();
+        bin_finish(tiled);
+    }
+    // The tree points (= current x) and their bounding box on its way to
+    // the host; work enqueued before bin_finish runs while the host waits.
+    void bin_begin() {
         HIP_OK(hipMemcpyAsync(xtree, x, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
         compute_bbox(st, xtree, n, dim, d_bbox, d_bbtmp);
         d2h(st, h_bbox, d_bbox, 6);
-        HIP_OK(hipStreamSynchronize(st));
+        HIP_OK(hipEventRecord(ev_bbox, st));
+    }
+    // The cell grid from the bounding box, then the binning (phase 0 times
+    // this part).
+    void bin_finish(bool tiled = false) {
+        HIP_OK(hipEventSynchronize(ev_bbox));
+        ph_begin(0);
         grid_setup(h_bbox, n, dim, h, grid);
         if (tiled) {
             Grid tg = grid;
@@ -832,10 +847,12 @@ struct ssasdpd_engine {
             // coalesced (cell order shares candidate cells but scatters the
             // writes, and measured slower); each sorted list is finished (r,
             // D_ij with the current rho, rows) in the same kernel
-            // (with the RDME on and few species the finish also forms Ddiag)
+            // (with the RDME on and few species the finish also forms Ddiag;
+            // then nothing reads the per-edge D_ij and neighbour types but
+            // the event loop, which has them in its rows)
             const bool fin_diag = rdme_on() && S <= FIN_MAX_SPECIES && S * T <= FIN_MAX_DMAT;
-            const FinishArgs fa{nr, Dij, ntype, erow, mass, rho, type, rec_ready ? rec : nullptr,
-                                fin_diag ? Ddiag : nullptr, Dmat, S, T};
+            const FinishArgs fa{nr, fin_diag ? nullptr : Dij, fin_diag ? nullptr : ntype, erow, mass, rho, type,
+                                rec_ready ? rec : nullptr, fin_diag ? Ddiag : nullptr, Dmat, S, T};
             // LDS list slots sized by the previous search's longest list
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
                       deg, derr, nullptr, &fa, last_max_deg);
@@ -937,7 +954,13 @@ struct ssasdpd_engine {
     void particle_phase(unsigned step) {
         const bool moving = static_domain == 0;
         rec_ready = false;
-        if (step == 0 || moving) {
+        // after step 0 on a moving domain the tree points are copied before
+        // the predictor moves the particles, and the predictor runs while
+        // the host waits for their bounding box
+        const bool overlap = moving && step > 0;
+        if (overlap) {
+            bin_begin();
+        } else if (step == 0 || moving) {
             bin_tree_points();
             // substep 0 lists (simulate.cpp:61-63).  On a moving domain the
             // reference rebuilds them at every step, but after step 0 nothing
@@ -949,6 +972,7 @@ struct ssasdpd_engine {
         ph_begin(2);
         sdpd_predict(st, sa);                      // substep 0 (+ the gather records)
         ph_end(2);
+        if (overlap) bin_finish();
         rec_ready = true;
         if (step > 0 && moving) {
             search();                              // substep 1 lists (simulate.cpp:121-123)

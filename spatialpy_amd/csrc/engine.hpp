@@ -172,7 +172,7 @@ struct EdgeRec {
 constexpr int FIN_MAX_SPECIES = 4;
 constexpr int FIN_MAX_DMAT = 64;
 struct FinishArgs {
-    double *r, *Dij;
+    double *r, *Dij;         // Dij, ntype: column-major per edge, or null (not formed)
     int *ntype;
     EdgeRec *erow;
     const double *mass, *rho;

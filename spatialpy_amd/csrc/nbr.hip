@@ -854,8 +854,8 @@ k_fr_search(const double *__restrict__ q, int m, const double *__restrict__ spts
                         }
                         const size_t e = (size_t)k * m + qi;
                         fin.r[e] = rr;
-                        fin.Dij[e] = D;
-                        fin.ntype[e] = tj[u];
+                        if (fin.Dij) fin.Dij[e] = D;
+                        if (fin.ntype) fin.ntype[e] = tj[u];
                         fin.erow[(size_t)qi * W + k] = EdgeRec{jj[u], tj[u], D};
                         if (fin.Ddiag) {
 #pragma unroll
