@@ -590,6 +590,30 @@ def bench_cylinder(args):
     if full is not None:
         out["full_run"] = full
     eng.close()
+    if not args.no_fp32_leg:
+        # the same window with FP32 pair terms (FP64 integration and sums):
+        # the north-star form of the pair forces, beside the reference's
+        # FP64 arithmetic that value is measured with
+        import copy
+        sa32 = copy.copy(sa)
+        sa32.flags = sa.flags | _abi.FLAG_FP32_FORCES
+        e32 = Engine(sa32)
+        e32.reset(seed=12345)
+        e32.advance(first)
+        e32.state()
+        p0 = e32.phase_times()
+        t = time.perf_counter()
+        e32.advance(K)
+        s32 = time.perf_counter() - t
+        p1 = e32.phase_times()
+        e32.close()
+        _progress(f"fp32 pair forces: {1e3 * s32 / K:.4f} ms/step")
+        out["fp32_forces"] = {"value": n * K / s32, "ms_per_step": round(1e3 * s32 / K, 4),
+                              "phase_ms_per_step": {k: round((p1[k][0] - p0[k][0]) / K, 4) for k in p1},
+                              "note": "same run and window, FP32 pair terms from FP64-formed dx and dv with FP64 "
+                                      "sums and FP64 integration (within 1e-5 of the FP64 oracle in "
+                                      "tests/test_gpu_forces_fp32.py); value above keeps the reference's FP64 "
+                                      "arithmetic"}
     if not args.no_cpu_baseline:
         _progress(f"cpu baseline: {cpu_steps} reference steps from step {first}")
         out["cpu_baseline"] = cpu_baseline(sa, state, first, cpu_steps)
@@ -716,6 +740,8 @@ def main():
     ap.add_argument("--no-full-run", action="store_true",
                     help="cylinder: stop after the timed window instead of finishing the 2,000-step run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32-leg", action="store_true",
+                    help="cylinder: skip timing the same window with FP32 pair terms")
     ap.add_argument("--ann-order", action="store_true", help="ANN tie order on the moving domain")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate leg (training on the plain run, the run with the network attached)")

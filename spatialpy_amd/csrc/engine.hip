@@ -108,6 +108,11 @@ struct ssasdpd_engine {
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
     hipEvent_t ev_bbox = nullptr;   // the bounding box has reached the host
+    // a step's queue state and particle errors, read into pinned memory
+    // behind the event loop's launch (ev_q: they have arrived)
+    QueueState *h_q = nullptr;
+    DevErr *h_pe = nullptr;
+    hipEvent_t ev_q = nullptr;
     // A renumber inside the look-ahead: the event loop beside it reads the
     // types from rtype (a copy made before it, ev_rt) while the particle
     // arrays are permuted; the counts are permuted after it on the engine
@@ -276,6 +281,10 @@ struct ssasdpd_engine {
         la_done = nullptr;
         if (ev_bbox) (void)hipEventDestroy(ev_bbox);
         ev_bbox = nullptr;
+        if (ev_q) (void)hipEventDestroy(ev_q);
+        ev_q = nullptr;
+        h_q = nullptr;      // inside h_bbox's block
+        h_pe = nullptr;
         if (ev_rt) (void)hipEventDestroy(ev_rt);
         if (ev_map) (void)hipEventDestroy(ev_map);
         ev_rt = ev_map = nullptr;
@@ -469,6 +478,7 @@ struct ssasdpd_engine {
             HIP_OK(hipEventCreateWithFlags(&la_done, hipEventDisableTiming));
         }
         HIP_OK(hipEventCreateWithFlags(&ev_bbox, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_q, hipEventDisableTiming));
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++) {
                 HIP_OK(hipEventCreate(&ev_a[p][k]));
@@ -487,7 +497,16 @@ struct ssasdpd_engine {
         if (ps) deg_alt = dalloc<int>(n);
         sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
         d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
-        HIP_OK(hipHostMalloc((void **)&h_bbox, 8 * sizeof(double), hipHostMallocDefault));
+        {
+            // one pinned block: the bounding box, the queue state and the
+            // particle errors read back each step
+            constexpr size_t o_q = 64, o_pe = o_q + ((sizeof(QueueState) + 63) / 64) * 64;
+            void *blk = nullptr;
+            HIP_OK(hipHostMalloc(&blk, o_pe + ((sizeof(DevErr) + 63) / 64) * 64, hipHostMallocDefault));
+            h_bbox = (double *)blk;
+            h_q = (QueueState *)((char *)blk + o_q);
+            h_pe = (DevErr *)((char *)blk + o_pe);
+        }
         derr = dalloc<DevErr>(1);
         d_psum = dalloc<double>(2);
         kd.parent = dalloc<int>(2 * (size_t)n + 2); kd.depth = dalloc<int>(2 * (size_t)n + 2);
@@ -1139,26 +1158,27 @@ struct ssasdpd_engine {
         bool la = false;
         if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
             RdmeArgs ra = rdme_args(step);
+            bool want = false, qpend = false;
             if (static_domain == 0 || !rdme_initialized) {
                 ph_begin(3);
                 rdme_rates(st, ra);
                 rdme_queue_build(st, ra, scan_tmp, qwork);
                 ph_end(3);
-                QueueState q;
-                d2h(st, &q, qs, 1);
-                // this step's particle errors, read before a look-ahead can add the next step's
                 // a renumber step is prepared too, unless the event loop reads
                 // data-function values (those are permuted with the particles)
-                const bool want = lookahead && ps && Ddiag_alt && static_domain == 0 && !nn && step + 1 < nt &&
-                                  (!renumber_due(step + 1) || (rtype && n_df == 0));
-                DevErr pe{};
-                if (want) d2h(st, &pe, derr, 1);
-                HIP_OK(hipStreamSynchronize(st));
-                if (q.initialized == 1) rdme_initialized = true;
-                la = want && pe.code == 0;
+                want = lookahead && ps && Ddiag_alt && static_domain == 0 && !nn && step + 1 < nt &&
+                       (!renumber_due(step + 1) || (rtype && n_df == 0));
+                // the queue state, and this step's particle errors (read before
+                // a look-ahead can add the next step's); the event loop, which
+                // checks the queue state itself, is launched before the host
+                // waits for them
+                d2h(st, h_q, qs, 1);
+                if (want) d2h(st, h_pe, derr, 1);
+                HIP_OK(hipEventRecord(ev_q, st));
+                qpend = true;
             }
-            if (la && renumber_due(step + 1)) {
-                // the look-ahead permutes type beside this event loop
+            if (want && renumber_due(step + 1)) {
+                // a look-ahead would permute type beside this event loop
                 HIP_OK(hipMemcpyAsync(rtype, type, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
                 HIP_OK(hipEventRecord(ev_rt, st));
                 ra.type = rtype;
@@ -1166,6 +1186,11 @@ struct ssasdpd_engine {
             ph_begin(4);
             rdme_take_step(st, ra);
             ph_end(4);
+            if (qpend) {
+                HIP_OK(hipEventSynchronize(ev_q));
+                if (h_q->initialized == 1) rdme_initialized = true;
+                la = want && h_pe->code == 0;
+            }
         }
         if (la) {
             const bool ren = renumber_due(step + 1);
