@@ -163,7 +163,9 @@ __device__ __forceinline__ double chem_rhs(const SdpdArgs &a, int i, int r) {
 // CHEM: the reaction fluxes are added here; otherwise k_chem adds them
 // afterwards (models with custom deterministic rates), to the same Q in the
 // same order.
-template <typename Real, bool CHEM>
+// SPECIES: the model has deterministic species (without them the species
+// flux and its registers drop out: more wavefronts hide the gathers).
+template <typename Real, bool CHEM, bool SPECIES = true>
 __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     const int chunk = xcd_chunk((a.n + 255) / 256);
     if (chunk < 0) return;
@@ -183,8 +185,8 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     double Fbp[3] = {a.Fbp[i], a.Fbp[n + i], a.Fbp[2 * n + i]};
     double Frho = a.Frho[i];
     constexpr int MAXSC = 16;
-    double Q[MAXSC];
-    const int Sc = a.Sc < MAXSC ? a.Sc : MAXSC;
+    double Q[SPECIES ? MAXSC : 1];
+    const int Sc = SPECIES ? (a.Sc < MAXSC ? a.Sc : MAXSC) : 0;
     for (int s = 0; s < Sc; s++) Q[s] = a.Q[(size_t)i * a.Sc + s];
     const int ti = a.type[i];
     const bool momentum = a.static_domain == 0;
@@ -420,6 +422,11 @@ void sdpd_forces(hipStream_t s, const SdpdArgs &a) {
         if (a.fp32) hipLaunchKernelGGL((k_forces<float, false>), grid1(a.n), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_forces<double, false>), grid1(a.n), dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_chem, grid1(a.n), dim3(256), 0, s, a);
+        return;
+    }
+    if (a.Sc == 0) {
+        if (a.fp32) hipLaunchKernelGGL((k_forces<float, false, false>), grid1(a.n), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_forces<double, false, false>), grid1(a.n), dim3(256), 0, s, a);
         return;
     }
     if (a.fp32) hipLaunchKernelGGL((k_forces<float, true>), grid1(a.n), dim3(256), 0, s, a);
