@@ -857,9 +857,13 @@ struct ssasdpd_engine {
 
     // Fixed-radius search from query = current x against the binned tree
     // points, then r and D_ij with the current rho (find_neighbors).
-    void search() {
-        ph_begin(1);
+    // forces_step >= 0: the step's plain pair forces are launched behind
+    // each search, before the host reads its overflow flag (the pair kernel
+    // starts its sums from the predictor's values, so the launch after a
+    // re-search gives the result of a single one)
+    void search(int forces_step = -1) {
         for (int attempt = 0; attempt < 4; attempt++) {
+            ph_begin(1);
             HIP_OK(hipMemsetAsync(&derr->overflow, 0, sizeof(int), st));
             HIP_OK(hipMemsetAsync(&derr->max_deg, 0, sizeof(int), st));
             // queries in particle order: each wave's list writes are
@@ -875,13 +879,18 @@ struct ssasdpd_engine {
             // LDS list slots sized by the previous search's longest list
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
                       deg, derr, nullptr, &fa, last_max_deg);
+            ph_end(1);
+            if (forces_step >= 0) {
+                ph_begin(5);
+                sdpd_forces(st, sdpd_args((unsigned)forces_step));
+                ph_end(5);
+            }
             DevErr e = read_err();
             last_max_deg = e.max_deg;
             max_deg_seen = std::max(max_deg_seen, e.max_deg);
             if (!e.overflow) break;
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
-        ph_end(1);
         lists_built = true;
     }
 
@@ -993,25 +1002,35 @@ struct ssasdpd_engine {
         ph_end(2);
         if (overlap) bin_finish();
         rec_ready = true;
-        if (step > 0 && moving) {
-            search();                              // substep 1 lists (simulate.cpp:121-123)
-            sa = sdpd_args(step);
-        }
-        ph_begin(5);
         // attached: checks run while the network is on, or always when a
         // failed check only pauses it until the next passing one (re-arm)
         const bool nn_on = nn && step > 0 && static_domain == 0 && (nn_active || nn_rearm);
+        const bool nn_step = nn_on && nn_active && step % nn_check_every != 0;
+        bool forces_done = false;
+        if (step > 0 && moving) {
+            // substep 1 lists (simulate.cpp:121-123); a plain step's pair
+            // forces go behind the search
+            search(nn_step ? -1 : (int)step);
+            sa = sdpd_args(step);
+            forces_done = !nn_step;
+        }
         if (nn_on) nn_ensure();
-        if (nn_on && nn_active && step % nn_check_every != 0) {
+        if (nn_step) {
             // surrogate step: the pair kernel supplies Fbp, Frho and Q only,
             // the network the accelerations
+            ph_begin(5);
             sa.skip_F = 1;
             sdpd_forces(st, sa);
             sa.skip_F = 0;
             nn_predict(F, true);
+            ph_end(5);
             nn_steps++;
         } else {
-            sdpd_forces(st, sa);                   // substep 1
+            if (!forces_done) {
+                ph_begin(5);
+                sdpd_forces(st, sa);               // substep 1
+                ph_end(5);
+            }
             if (nn_rec && step > 0 && static_domain == 0 && step % nn_rec_every == 0)
                 record_sample(nn_rec, nn_rec_max, nn_rec_seed++);
             if (nn_on && step % nn_check_every == 0) {
@@ -1043,7 +1062,6 @@ struct ssasdpd_engine {
                 }
             }
         }
-        ph_end(5);
         ph_begin(2);
         sdpd_correct(st, sa);                      // substep 2
         sdpd_bvf(st, sa);

@@ -181,13 +181,16 @@ __global__ void __launch_bounds__(256) k_forces(SdpdArgs a) {
     const Real dvti[3] = {(Real)(vi[0] - vti[0]), (Real)(vi[1] - vti[1]), (Real)(vi[2] - vti[2])};
     const Real rhoi = (Real)a.rho[i], mi = (Real)a.mass[i], nui = (Real)a.nu[i];
     const Real Pi = P0 * (rhoi / rho0 - (Real)1.0);
-    double F[3] = {a.F[i], a.F[n + i], a.F[2 * n + i]};
-    double Fbp[3] = {a.Fbp[i], a.Fbp[n + i], a.Fbp[2 * n + i]};
-    double Frho = a.Frho[i];
+    // the sums start where the predictor resets them (F = g, Fbp, Frho and
+    // Q = 0: simulate.cpp:92-105), not from the arrays, so a second launch
+    // on the same lists (after a re-search) gives the same result
+    double F[3] = {a.g[0], a.g[1], a.g[2]};
+    double Fbp[3] = {0.0, 0.0, 0.0};
+    double Frho = 0.0;
     constexpr int MAXSC = 16;
     double Q[SPECIES ? MAXSC : 1];
     const int Sc = SPECIES ? (a.Sc < MAXSC ? a.Sc : MAXSC) : 0;
-    for (int s = 0; s < Sc; s++) Q[s] = a.Q[(size_t)i * a.Sc + s];
+    for (int s = 0; s < Sc; s++) Q[s] = 0.0;
     const int ti = a.type[i];
     const bool momentum = a.static_domain == 0;
     const int dg = a.deg[i];
