@@ -108,15 +108,16 @@ struct ssasdpd_engine {
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
     hipEvent_t ev_bbox = nullptr;   // the bounding box has reached the host
-    // a step's queue state and particle errors, read into pinned memory
-    // behind the event loop's launch (ev_q: they have arrived)
+    // a static domain's queue state, read into pinned memory behind the
+    // event loop's launch (ev_q: it has arrived)
     QueueState *h_q = nullptr;
-    DevErr *h_pe = nullptr;
     hipEvent_t ev_q = nullptr;
-    // A renumber inside the look-ahead: the event loop beside it reads the
-    // types from rtype (a copy made before it, ev_rt) while the particle
-    // arrays are permuted; the counts are permuted after it on the engine
-    // stream, by the order kept in rmap (ev_map), through xbuf.
+    // A look-ahead starts behind ev_rt, recorded on the engine stream after
+    // this step's particle phase (which may have run there).  A renumber
+    // inside the look-ahead: the event loop beside it reads the types from
+    // rtype (a copy made before ev_rt) while the particle arrays are
+    // permuted; the counts are permuted after it on the engine stream, by
+    // the order kept in rmap (ev_map), through xbuf.
     int *rtype = nullptr, *rmap = nullptr;
     uint32_t *xbuf = nullptr;
     hipEvent_t ev_rt = nullptr, ev_map = nullptr;
@@ -163,6 +164,7 @@ struct ssasdpd_engine {
     void *kd_work = nullptr;
     size_t kd_work_sz = 0;
     void *scan_tmp = nullptr;
+    void *scan_tmp_q = nullptr;   // the queue build's (it can run beside a binning)
     // chemistry constants
     Rxn *rx = nullptr;
     // expression-VM programs of kind-4 reactions, data functions [n][n_df]
@@ -192,7 +194,9 @@ struct ssasdpd_engine {
     ssasdpd_event *d_trace = nullptr;
     uint64_t trace_cap = 0;
     // errors
-    DevErr *derr = nullptr;
+    // the particle phase's error record; a look-ahead writes the other one
+    // (swapped in with deg and erow), so each step's check reads its own
+    DevErr *derr = nullptr, *derr_alt = nullptr;
     // progress
     uint32_t cur = 0;
     bool lists_built = false;
@@ -284,7 +288,6 @@ struct ssasdpd_engine {
         if (ev_q) (void)hipEventDestroy(ev_q);
         ev_q = nullptr;
         h_q = nullptr;      // inside h_bbox's block
-        h_pe = nullptr;
         if (ev_rt) (void)hipEventDestroy(ev_rt);
         if (ev_map) (void)hipEventDestroy(ev_map);
         ev_rt = ev_map = nullptr;
@@ -300,6 +303,9 @@ struct ssasdpd_engine {
         kd_work = nullptr;
         if (scan_tmp) (void)hipFree(scan_tmp);
         scan_tmp = nullptr;
+        if (scan_tmp_q) (void)hipFree(scan_tmp_q);
+        scan_tmp_q = nullptr;
+        dfree(derr_alt);
         dfree(vm_code); dfree(vm_k); dfree(dfn);
         dfree(rx); dfree(jcN); dfree(irN); dfree(prN); dfree(jcG); dfree(irG); dfree(Ndense); dfree(Dmat); dfree(bc);
         dfree(xx); dfree(rrate); dfree(srrate); dfree(sdrate); dfree(Ddiag); dfree(qtime); dfree(tmin); dfree(targ);
@@ -498,16 +504,16 @@ struct ssasdpd_engine {
         sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
         d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
         {
-            // one pinned block: the bounding box, the queue state and the
-            // particle errors read back each step
-            constexpr size_t o_q = 64, o_pe = o_q + ((sizeof(QueueState) + 63) / 64) * 64;
+            // one pinned block: the bounding box and the queue state read
+            // back each step
+            constexpr size_t o_q = 64;
             void *blk = nullptr;
-            HIP_OK(hipHostMalloc(&blk, o_pe + ((sizeof(DevErr) + 63) / 64) * 64, hipHostMallocDefault));
+            HIP_OK(hipHostMalloc(&blk, o_q + ((sizeof(QueueState) + 63) / 64) * 64, hipHostMallocDefault));
             h_bbox = (double *)blk;
             h_q = (QueueState *)((char *)blk + o_q);
-            h_pe = (DevErr *)((char *)blk + o_pe);
         }
         derr = dalloc<DevErr>(1);
+        if (ps) derr_alt = dalloc<DevErr>(1);
         d_psum = dalloc<double>(2);
         kd.parent = dalloc<int>(2 * (size_t)n + 2); kd.depth = dalloc<int>(2 * (size_t)n + 2);
         kd.cd = dalloc<int>(2 * (size_t)n + 2); kd.cv = dalloc<double>(2 * (size_t)n + 2);
@@ -516,6 +522,7 @@ struct ssasdpd_engine {
         HIP_OK(hipMalloc(&kd_work, kd_work_sz));
         size_t scan_sz = std::max(scan_tmp_bytes(4LL * n + (1 << 20) + 8), scan_tmp_bytes(n + 8));
         HIP_OK(hipMalloc(&scan_tmp, scan_sz));
+        HIP_OK(hipMalloc(&scan_tmp_q, scan_tmp_bytes(n + 8)));
         alloc_lists(dim == 3 ? 40 : (dim == 2 ? 16 : 4));
 
         // constants
@@ -676,6 +683,7 @@ struct ssasdpd_engine {
         }
         DevErr e0{0, INT32_MAX, 0, 0, 0};
         h2d(st, derr, &e0, 1);
+        if (derr_alt) h2d(st, derr_alt, &e0, 1);
         rng_kind = (r && r->kind == 1) ? 1 : 0;
         seed = r ? r->seed : 0;
         if (rng_kind == 1) {
@@ -710,9 +718,9 @@ struct ssasdpd_engine {
         for (int p = 0; p < NPH; p++) { ph_ms[p] = 0; ph_launch[p] = 0; pending[p][0] = pending[p][1] = false; }
     }
 
-    DevErr read_err() {
+    DevErr read_err(const DevErr *from = nullptr) {
         DevErr e;
-        d2h(st, &e, derr, 1);
+        d2h(st, &e, from ? from : derr, 1);
         HIP_OK(hipStreamSynchronize(st));
         return e;
     }
@@ -937,11 +945,10 @@ struct ssasdpd_engine {
         return a;
     }
 
-    // particles: also report the particle phase's errors (off when a
-    // look-ahead has already run the next step's particle phase).
-    int check(unsigned step, bool particles = true) {
-        DevErr e{};
-        if (particles) e = read_err();
+    // The step's particle-phase errors (from its error record: `eb`, else
+    // the current one), then its event loop's.
+    int check(unsigned step, const DevErr *eb = nullptr) {
+        const DevErr e = read_err(eb);
         if (e.code == SSASDPD_ERR_CAPACITY && e.crowded > 0) {
             stats.bad_particle = -1;
             char buf[320];
@@ -1153,7 +1160,7 @@ struct ssasdpd_engine {
         particle_phase(0);
         RdmeArgs ra = rdme_args(0);
         rdme_rates(st, ra);
-        rdme_queue_build(st, ra, scan_tmp, qwork);
+        rdme_queue_build(st, ra, scan_tmp_q, qwork);
         QueueState q;
         d2h(st, &q, qs, 1);
         HIP_OK(hipStreamSynchronize(st));
@@ -1176,30 +1183,37 @@ struct ssasdpd_engine {
         bool la = false;
         if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
             RdmeArgs ra = rdme_args(step);
-            bool want = false, qpend = false;
-            if (static_domain == 0 || !rdme_initialized) {
+            const bool build = static_domain == 0 || !rdme_initialized;
+            // A look-ahead runs the next step's particle phase beside this
+            // step's rates, queue build and event loop: they read vol, Ddiag,
+            // deg and the rows of this step (the look-ahead writes the other
+            // buffers), the types from a copy when it renumbers, and not the
+            // particle state; its errors go to the other error record.  A
+            // renumber step is prepared too, unless the event loop reads
+            // data-function values (those are permuted with the particles).
+            const bool want = build && lookahead && ps && Ddiag_alt && derr_alt && static_domain == 0 && !nn &&
+                              step + 1 < nt && (!renumber_due(step + 1) || (rtype && n_df == 0));
+            if (want) {
+                if (renumber_due(step + 1)) {
+                    HIP_OK(hipMemcpyAsync(rtype, type, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+                    ra.type = rtype;
+                }
+                HIP_OK(hipEventRecord(ev_rt, st));
+            }
+            bool qpend = false;
+            if (build) {
                 ph_begin(3);
                 rdme_rates(st, ra);
-                rdme_queue_build(st, ra, scan_tmp, qwork);
+                rdme_queue_build(st, ra, scan_tmp_q, qwork);
                 ph_end(3);
-                // a renumber step is prepared too, unless the event loop reads
-                // data-function values (those are permuted with the particles)
-                want = lookahead && ps && Ddiag_alt && static_domain == 0 && !nn && step + 1 < nt &&
-                       (!renumber_due(step + 1) || (rtype && n_df == 0));
-                // the queue state, and this step's particle errors (read before
-                // a look-ahead can add the next step's); the event loop, which
-                // checks the queue state itself, is launched before the host
-                // waits for them
-                d2h(st, h_q, qs, 1);
-                if (want) d2h(st, h_pe, derr, 1);
-                HIP_OK(hipEventRecord(ev_q, st));
-                qpend = true;
-            }
-            if (want && renumber_due(step + 1)) {
-                // a look-ahead would permute type beside this event loop
-                HIP_OK(hipMemcpyAsync(rtype, type, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-                HIP_OK(hipEventRecord(ev_rt, st));
-                ra.type = rtype;
+                if (static_domain) {
+                    // whether the queue holds an event (a static domain then
+                    // runs its event loops from here on); the event loop,
+                    // which checks the queue state itself, is launched first
+                    d2h(st, h_q, qs, 1);
+                    HIP_OK(hipEventRecord(ev_q, st));
+                    qpend = true;
+                }
             }
             ph_begin(4);
             rdme_take_step(st, ra);
@@ -1207,8 +1221,8 @@ struct ssasdpd_engine {
             if (qpend) {
                 HIP_OK(hipEventSynchronize(ev_q));
                 if (h_q->initialized == 1) rdme_initialized = true;
-                la = want && h_pe->code == 0;
             }
+            la = want;
         }
         if (la) {
             const bool ren = renumber_due(step + 1);
@@ -1216,9 +1230,10 @@ struct ssasdpd_engine {
             std::swap(erow, erow_alt);
             std::swap(Ddiag, Ddiag_alt);
             std::swap(vol, vol_alt);
+            std::swap(derr, derr_alt);
             std::swap(st, ps);
+            HIP_OK(hipStreamWaitEvent(st, ev_rt, 0));
             if (ren) {
-                HIP_OK(hipStreamWaitEvent(st, ev_rt, 0));
                 const int *m = renumber_map();
                 HIP_OK(hipMemcpyAsync(rmap, m, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
                 HIP_OK(hipEventRecord(ev_map, st));
@@ -1235,7 +1250,9 @@ struct ssasdpd_engine {
             }
             la_pending = true;
         }
-        int rc = check(step, !la);
+        // this step's particle phase wrote the error record now in derr_alt
+        // when a look-ahead took derr
+        int rc = check(step, la ? derr_alt : nullptr);
         if (rc && la) {
             // the next step's particle phase is still running on ps: let it
             // finish so that state reads on st do not race it
@@ -1333,7 +1350,7 @@ struct ssasdpd_engine {
                 if (static_domain && b_init[b]) continue;
                 RdmeArgs ra = batch_args(step, b);
                 rdme_rates(st, ra);
-                rdme_queue_build(st, ra, scan_tmp, qwork);
+                rdme_queue_build(st, ra, scan_tmp_q, qwork);
                 any = true;
             }
             ph_end(3);

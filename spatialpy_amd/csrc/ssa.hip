@@ -187,8 +187,9 @@ __global__ void k_rates(RdmeArgs a) {
     for (int s = 0; s < S; s++) xv[s] = xx[(size_t)i * S + s];
     double sr = 0.0;
     const double *df = a.data_fn ? a.data_fn + (size_t)i * a.n_df : nullptr;
+    // mass / rho of this step, formed by the particle phase (simulate_rdme.cpp:123)
+    const double vol = a.vol[i];
     for (int j = 0; j < R; j++) {
-        double vol = a.mass[i] / a.rho[i];
         const Rxn re = a.rx[j];
         double p;
         if (VM && re.kind == 4)   // simulate_rdme.cpp:124 evaluates at t = 0.0
