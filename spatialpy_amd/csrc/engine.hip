@@ -108,10 +108,15 @@ struct ssasdpd_engine {
     hipEvent_t la_done = nullptr;
     bool la_pending = false;
     hipEvent_t ev_bbox = nullptr;   // the bounding box has reached the host
+    hipEvent_t ev_srch = nullptr;   // a search's error record has reached the host
     // a static domain's queue state, read into pinned memory behind the
     // event loop's launch (ev_q: it has arrived)
     QueueState *h_q = nullptr;
     hipEvent_t ev_q = nullptr;
+    // a step's error record and queue state, read back together after its
+    // event loop (the same pinned block)
+    DevErr *h_err = nullptr;
+    QueueState *h_qc = nullptr;
     // A look-ahead starts behind ev_rt, recorded on the engine stream after
     // this step's particle phase (which may have run there).  A renumber
     // inside the look-ahead: the event loop beside it reads the types from
@@ -241,6 +246,7 @@ struct ssasdpd_engine {
     void ph_begin(int p) {
         if (!ev_a[p][0]) return;
         int k = ev_n[p] & 1;
+        if (pending[p][k]) ph_take(p, k);   // a phase timed twice in one step
         HIP_OK(hipEventRecord(ev_a[p][k], st));
     }
     void ph_end(int p) {
@@ -249,18 +255,33 @@ struct ssasdpd_engine {
         HIP_OK(hipEventRecord(ev_b[p][k], st));
         ev_n[p]++;
         pending[p][k] = true;
+        ahead[p][k] = in_lookahead;
     }
     bool pending[NPH][2] = {};
-    void ph_collect() {
+    // ahead: recorded by a look-ahead (the next step's particle phase on the
+    // second stream), which the host does not wait for within its step
+    bool ahead[NPH][2] = {};
+    bool in_lookahead = false;
+    void ph_take(int p, int k) {
+        float ms = 0;
+        HIP_OK(hipEventSynchronize(ev_b[p][k]));
+        HIP_OK(hipEventElapsedTime(&ms, ev_a[p][k], ev_b[p][k]));
+        ph_ms[p] += ms;
+        ph_launch[p]++;
+        pending[p][k] = false;
+    }
+    // skip_ahead: leave the phases of a look-ahead launched in this step for
+    // the next call (by then the engine stream has waited for them), so the
+    // host does not wait here for the next step's particle phase
+    void ph_collect(bool skip_ahead = false) {
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++)
                 if (pending[p][k]) {
-                    float ms = 0;
-                    HIP_OK(hipEventSynchronize(ev_b[p][k]));
-                    HIP_OK(hipEventElapsedTime(&ms, ev_a[p][k], ev_b[p][k]));
-                    ph_ms[p] += ms;
-                    ph_launch[p]++;
-                    pending[p][k] = false;
+                    if (skip_ahead && ahead[p][k]) {
+                        ahead[p][k] = false;
+                        continue;
+                    }
+                    ph_take(p, k);
                 }
     }
 
@@ -285,9 +306,12 @@ struct ssasdpd_engine {
         la_done = nullptr;
         if (ev_bbox) (void)hipEventDestroy(ev_bbox);
         ev_bbox = nullptr;
+        if (ev_srch) (void)hipEventDestroy(ev_srch);
+        ev_srch = nullptr;
         if (ev_q) (void)hipEventDestroy(ev_q);
         ev_q = nullptr;
-        h_q = nullptr;      // inside h_bbox's block
+        h_q = h_qc = nullptr;   // inside h_bbox's block
+        h_err = nullptr;
         if (ev_rt) (void)hipEventDestroy(ev_rt);
         if (ev_map) (void)hipEventDestroy(ev_map);
         ev_rt = ev_map = nullptr;
@@ -484,6 +508,7 @@ struct ssasdpd_engine {
             HIP_OK(hipEventCreateWithFlags(&la_done, hipEventDisableTiming));
         }
         HIP_OK(hipEventCreateWithFlags(&ev_bbox, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev_srch, hipEventDisableTiming));
         HIP_OK(hipEventCreateWithFlags(&ev_q, hipEventDisableTiming));
         for (int p = 0; p < NPH; p++)
             for (int k = 0; k < 2; k++) {
@@ -504,13 +529,16 @@ struct ssasdpd_engine {
         sorted = dalloc<int>(n); cell_of = dalloc<int>(n);
         d_bbox = dalloc<double>(8); d_bbtmp = dalloc<unsigned long long>(8);
         {
-            // one pinned block: the bounding box and the queue state read
-            // back each step
-            constexpr size_t o_q = 64;
+            // one pinned block: the bounding box, the queue state and the
+            // error record read back each step
+            constexpr size_t qsz = ((sizeof(QueueState) + 63) / 64) * 64;
+            constexpr size_t o_q = 64, o_qc = o_q + qsz, o_err = o_qc + qsz;
             void *blk = nullptr;
-            HIP_OK(hipHostMalloc(&blk, o_q + ((sizeof(QueueState) + 63) / 64) * 64, hipHostMallocDefault));
+            HIP_OK(hipHostMalloc(&blk, o_err + ((sizeof(DevErr) + 63) / 64) * 64, hipHostMallocDefault));
             h_bbox = (double *)blk;
             h_q = (QueueState *)((char *)blk + o_q);
+            h_qc = (QueueState *)((char *)blk + o_qc);
+            h_err = (DevErr *)((char *)blk + o_err);
         }
         derr = dalloc<DevErr>(1);
         if (ps) derr_alt = dalloc<DevErr>(1);
@@ -719,10 +747,9 @@ struct ssasdpd_engine {
     }
 
     DevErr read_err(const DevErr *from = nullptr) {
-        DevErr e;
-        d2h(st, &e, from ? from : derr, 1);
+        d2h(st, h_err, from ? from : derr, 1);
         HIP_OK(hipStreamSynchronize(st));
-        return e;
+        return *h_err;
     }
 
     // Bin the tree points (= current x), and build the kd-tree when list
@@ -888,15 +915,21 @@ struct ssasdpd_engine {
             fr_search(st, x, n, spts, sorted, cell_start, n, dim, h, grid, kd.valid ? &kd : nullptr, W, nidx, nr2,
                       deg, derr, nullptr, &fa, last_max_deg);
             ph_end(1);
+            // the host waits for the search's overflow flag only, not for
+            // the pair kernel behind it
+            d2h(st, h_err, derr, 1);
+            HIP_OK(hipEventRecord(ev_srch, st));
             if (forces_step >= 0) {
                 ph_begin(5);
                 sdpd_forces(st, sdpd_args((unsigned)forces_step));
                 ph_end(5);
             }
-            DevErr e = read_err();
+            HIP_OK(hipEventSynchronize(ev_srch));
+            const DevErr e = *h_err;
             last_max_deg = e.max_deg;
             max_deg_seen = std::max(max_deg_seen, e.max_deg);
             if (!e.overflow) break;
+            HIP_OK(hipStreamSynchronize(st));   // the pair kernel reads the lists freed here
             alloc_lists(((e.max_deg + 7) / 8) * 8 + 8);
         }
         lists_built = true;
@@ -948,7 +981,11 @@ struct ssasdpd_engine {
     // The step's particle-phase errors (from its error record: `eb`, else
     // the current one), then its event loop's.
     int check(unsigned step, const DevErr *eb = nullptr) {
-        const DevErr e = read_err(eb);
+        const bool q_on = S > 0 && !(flags & SSASDPD_FLAG_NO_RDME);
+        d2h(st, h_err, eb ? eb : derr, 1);
+        if (q_on) d2h(st, h_qc, qs, 1);
+        HIP_OK(hipStreamSynchronize(st));
+        const DevErr e = *h_err;
         if (e.code == SSASDPD_ERR_CAPACITY && e.crowded > 0) {
             stats.bad_particle = -1;
             char buf[320];
@@ -967,10 +1004,8 @@ struct ssasdpd_engine {
             return set_error(e.code, e.code == SSASDPD_ERR_NAN_STATE ? std::string("nan/inf state at ") + buf
                                                                         : std::string("NaN D_ij at ") + buf);
         }
-        if (S > 0 && !(flags & SSASDPD_FLAG_NO_RDME)) {
-            QueueState q;
-            d2h(st, &q, qs, 1);
-            HIP_OK(hipStreamSynchronize(st));
+        if (q_on) {
+            const QueueState q = *h_qc;
             stats.reactions = q.reactions;
             stats.diffusions = q.diffusions;
             stats.rng_words_used = q.rng_pos;
@@ -1081,8 +1116,8 @@ struct ssasdpd_engine {
         }
     }
 
-    void end_step() {
-        ph_collect();
+    void end_step(bool skip_ahead = false) {
+        ph_collect(skip_ahead);
         stats.ms_neighbors = ph_ms[0] + ph_ms[1];
         stats.ms_sdpd = ph_ms[2] + ph_ms[5];
         stats.ms_rates = ph_ms[3];
@@ -1232,6 +1267,7 @@ struct ssasdpd_engine {
             std::swap(vol, vol_alt);
             std::swap(derr, derr_alt);
             std::swap(st, ps);
+            in_lookahead = true;
             HIP_OK(hipStreamWaitEvent(st, ev_rt, 0));
             if (ren) {
                 const int *m = renumber_map();
@@ -1241,6 +1277,7 @@ struct ssasdpd_engine {
             }
             particle_phase(step + 1);
             HIP_OK(hipEventRecord(la_done, st));
+            in_lookahead = false;
             std::swap(st, ps);
             if (ren) {
                 // the counts, once this step's event loop is done with them
@@ -1258,7 +1295,7 @@ struct ssasdpd_engine {
             // finish so that state reads on st do not race it
             HIP_OK(hipStreamSynchronize(ps));
         }
-        end_step();
+        end_step(la);
         return rc;
     }
 
